@@ -1,0 +1,39 @@
+// One CAMBI instance of a picture size: the per-picture form of Cambi (src/cambi.hpp, src/cambi.cpp) over the gfx950
+// kernels, with its buffers, its own stream and the select on the device.
+#pragma once
+#include "host_common.hpp"
+
+namespace es {
+
+class CambiUnit {
+public:
+    CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues, int tile_h = 0);
+    ~CambiUnit();
+    CambiUnit(const CambiUnit &) = delete;
+    CambiUnit &operator=(const CambiUnit &) = delete;
+    // Reads the plane in stream order on `caller`; everything after the first kernel runs on the own stream.
+    void launch(const void *img, int pitch, int bytes, int shift, hipStream_t caller);
+    double finish();
+    void scales(es_cambi_scale out[kScales]) const;
+    void plane(int s, uint16_t *host) const;
+    void cvalues(int s, float *host) const;
+    const CambiInfo &info() const { return info_; }
+
+private:
+    struct Results {
+        double score;
+        unsigned long long sums[kScales];
+        unsigned int counts[kScales];
+        TopkJob jobs[kScales];
+    };
+    CambiInfo info_;
+    bool keep_;
+    int tile_h_, tiles_ = 0, base_tiles_ = 0;
+    DevBuf consts_, planes_, cvals_, list_, sums_, counts_, jobs_, hist_, hsum_, ticket_, pic_, first_, score_;
+    HostBuf stage_, results_;
+    hipStream_t own_ = nullptr;
+    hipEvent_t copied_ = nullptr;
+    bool pending_ = false;
+};
+
+}  // namespace es

@@ -1,0 +1,44 @@
+// Device helpers shared by the kernel files: sample reads, wave-wide scans and reductions, exact fixed point.
+#pragma once
+#include "es_internal.hpp"
+
+namespace es {
+
+__device__ __forceinline__ unsigned raw_sample(const PlaneIn &in, int x, int y)
+{
+    const size_t i = (size_t)y * in.pitch + x;
+    const unsigned v = in.bytes == 1 ? ((const uint8_t *)in.ptr)[i] : ((const uint16_t *)in.ptr)[i];
+    return v << in.shift;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave with DPP: shifts of 1, 2, 4, 8 inside rows of 16 lanes, then the
+// row sums broadcast from lane 15 to rows 1 and 3 and from lane 31 to rows 2 and 3.  Lanes without a source add 0.
+__device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// A positive c-value (at least 0.5) as an exact integer in units of 2^-24.
+__device__ __forceinline__ unsigned long long fixed24(float c) { return (unsigned long long)((double)c * 16777216.0); }
+
+// Histogram::add's bin (src/stats.hpp:16): 0.05-point bins over [-500, 500], rounded half away from zero.
+__device__ __forceinline__ int hist_bin(double pct)
+{
+    const double c = pct < -500.0 ? -500.0 : pct > 500.0 ? 500.0 : pct;
+    return (int)round((c - -500.0) / 0.05);
+}
+
+}  // namespace es

@@ -1,0 +1,59 @@
+// Job descriptors and launch wrappers of the window-energy, lasting-texture and collect kernels.
+#pragma once
+#include "es_internal.hpp"
+
+namespace es {
+
+// One (source, encode) pair for window_stats: both planes cropped to w x h (the source's pointer already offset to
+// its crop), luma brought to the 10-bit scale by `up`.  mid_r / mid_d receive M = 81 mid on the area whole windows
+// cover (either may be null); sums receives 12 exact sums per window.
+struct WsPair {
+    PlaneIn r, d;
+    int w, h, up;
+    int32_t *mid_r, *mid_d;
+    long long *sums;
+};
+
+// One lasting-texture job: ring[j] is the source mid plane of offset j - 6 from the centre frame.
+//   plain:  mid_d = the encode's plane of the centre, out = 5 sums per window
+//   source: st receives ST = SA + SB, out = [M_r(c) ST, SA SB, SA^2, SB^2] per window
+//   encode: mid_d and st are read, out = [M_d(c) ST] per window
+struct PersistJob {
+    const int32_t *ring[kRing];
+    const int32_t *mid_d;
+    int32_t *st;
+    int w, h;
+    long long *out;
+};
+
+void launch_window_stats(const WsPair *pairs, int npairs, int max_w, int max_h, int win, bool fp64, hipStream_t st);
+void launch_persist(const PersistJob *jobs, int njobs, int max_w, int max_h, int win, int kind, hipStream_t st);
+
+// Device collect of one (encode, frame): structure threshold, mask, histogram entries and frame totals; and of the
+// lasting texture of a centre frame.  See collect_kernels.hip.
+struct CollectJob {
+    const long long *sums;        // windows x 12
+    int windows;
+    uint8_t *mask;                // out: this frame's structure mask (windows)
+    unsigned long long *hist;     // the segment's 6 histograms: amount[3], matching[2], lasting (6 x 20001)
+    long long *tot_int;           // out: 12 exact totals: all[6], structured[6]
+    double *tot_orient;           // out: 4 totals: all_r, all_d, st_r, st_d of the oriented energy
+    double *quantile;             // out: the structure threshold (diagnostic)
+    // lasting texture of the centre frame (unused when psrc is null)
+    const long long *psrc;        // windows x 4 (batched) or windows x 5 (plain sums), see lasting_plain
+    const long long *penc;        // windows x 1 (batched)
+    const uint8_t *cmask;         // the centre frame's structure mask
+    unsigned long long *lhist;    // the centre segment's histograms (lasting at index 5)
+    long long *tot_lasting;       // out: [own, mine, count] exact
+    int lasting_plain;
+};
+
+// phase 0: the frame part of every job; phase 1: the lasting texture (after every job's phase 0 wrote its mask).
+void launch_collect(const CollectJob *jobs, int njobs, double *scratch, int max_windows, int phase, hipStream_t st);
+void launch_hist_quantiles(const unsigned long long *hists, int nhist, const double *qs, int nq, double *out,
+                           unsigned long long *counts, hipStream_t st);
+void launch_select_u64(const unsigned long long *keys, long long n, long long rank, unsigned long long *out,
+                       hipStream_t st);
+void launch_select_u32(const unsigned int *keys, long long n, long long rank, unsigned int *out, hipStream_t st);
+
+}  // namespace es

@@ -1,0 +1,283 @@
+// Detail, structure-tensor and lasting-texture window sums for gfx950.
+//
+// window_stats (src/kernels.cu:331-428; reference/encode_score/compare.py:77-108): per win x win window of a (source,
+// encode) pair, the energies of fine = y - box3 and mid = box3 - box9 (edges repeated), their cross products and the
+// structure tensor of box3's central differences.  Luma is an integer and the box weights are 1/9 and 1/81, so the
+// kernel works on exact integers F = 9 fine, M = 81 mid, GX = 9 gx, GY = 9 gy and writes exact window sums:
+//   [F_r^2, M_r^2, F_d^2, M_d^2, F_r F_d, M_r M_d, GX_r^2, GY_r^2, GX_r GY_r, GX_d^2, GY_d^2, GX_d GY_d]
+// The FP64 form does the same arithmetic in double (exact too, every value being an integer below 2^53); it is the
+// per-frame baseline the integer form is timed against.
+//
+// One wave per strip of 64 columns, moving down WR rows of windows with the running 3- and 9-sums of the last rows
+// in registers; each row of both pictures is staged through LDS with 4 columns of margin either side.
+//
+// persist (src/kernels.cu:430-451): per window of a centre frame, from the source's mid planes of the 6 frames either
+// side, SA = sum at offsets +-2, 4, 6, SB = sum at +-1, 3, 5 (6 * 81 times the reference's A and B), ST = SA + SB
+// (12 * 81 T), and the sums [M_r(c) ST, M_d(c) ST, SA SB, SA^2, SB^2].
+#include "device_util.hpp"
+#include "energy.hpp"
+
+namespace es {
+
+constexpr int kWsRows = 2;  // rows of windows per wave
+
+template <typename T>
+struct WsPic {
+    // running sums of the last rows, newest last
+    T h3c[3], h3l[3], h3r[3], h9[9], yv[5];
+    T s3c[5], s3l[4], s3r[4];
+};
+
+template <typename T, int N>
+__device__ __forceinline__ void push(T (&a)[N], T v)
+{
+#pragma unroll
+    for (int k = 0; k < N - 1; k++) a[k] = a[k + 1];
+    a[N - 1] = v;
+}
+
+__device__ __forceinline__ int in_sample(const PlaneIn &in, int x, int y)
+{
+    const size_t i = (size_t)y * in.pitch + x;
+    const unsigned v = in.bytes == 1 ? ((const uint8_t *)in.ptr)[i] : ((const uint16_t *)in.ptr)[i];
+    return (int)(v << in.shift);
+}
+
+template <typename A>
+__device__ __forceinline__ A reduce_lanes(A v, int width)
+{
+    for (int o = width >> 1; o; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// T: the arithmetic type (int or double); A: the accumulator type (long long or double).
+template <int WIN, typename T, typename A>
+__global__ __launch_bounds__(256) void window_stats_kernel(const WsPair *pairs)
+{
+    __shared__ int rowbuf[2][4][2][64 + 8];  // double-buffered rows, per wave and picture
+    const WsPair &P = pairs[blockIdx.z];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int w = P.w, h = P.h, ww = w / WIN, wh = h / WIN, W = ww * WIN;
+    const int x0 = (blockIdx.x * 4 + wave) * 64, wy0 = blockIdx.y * kWsRows;
+    if (wy0 >= wh || blockIdx.x * 256 >= W) return;  // uniform over the block
+    const int x = x0 + lane;
+    const bool inside = x < W;
+    const int y0 = wy0 * WIN, y1 = min(wy0 + kWsRows, wh) * WIN;
+    WsPic<T> pic[2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) pic[p].h3c[k] = pic[p].h3l[k] = pic[p].h3r[k] = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) pic[p].h9[k] = 0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) pic[p].yv[k] = pic[p].s3c[k] = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) pic[p].s3l[k] = pic[p].s3r[k] = 0;
+    }
+    A acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) acc[k] = 0;
+    const PlaneIn *in[2] = {&P.r, &P.d};
+    int32_t *mid[2] = {P.mid_r, P.mid_d};
+    int buf = 0;
+    for (int r = y0 - 4; r < y1 + 4; r++) {
+        const int yy = min(max(r, 0), h - 1);
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            rowbuf[buf][wave][p][lane + 4] = in_sample(*in[p], min(x, w - 1), yy) << P.up;
+            if (lane < 8) {
+                const int xh = lane < 4 ? x0 - 4 + lane : x0 + 60 + lane;
+                rowbuf[buf][wave][p][lane < 4 ? lane : lane + 64] = in_sample(*in[p], min(max(xh, 0), w - 1), yy) << P.up;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            T v[9];
+#pragma unroll
+            for (int d = 0; d < 9; d++) v[d] = (T)rowbuf[buf][wave][p][lane + d];  // columns x - 4 .. x + 4
+            T s9 = 0;
+#pragma unroll
+            for (int d = 0; d < 9; d++) s9 += v[d];
+            WsPic<T> &q = pic[p];
+            push(q.h3l, v[2] + v[3] + v[4]);
+            push(q.h3c, v[3] + v[4] + v[5]);
+            push(q.h3r, v[4] + v[5] + v[6]);
+            push(q.h9, s9);
+            push(q.yv, v[4]);
+            // 3x3 sums of the row before this one (complete now that this row is in)
+            push(q.s3c, q.h3c[0] + q.h3c[1] + q.h3c[2]);
+            push(q.s3l, q.h3l[0] + q.h3l[1] + q.h3l[2]);
+            push(q.s3r, q.h3r[0] + q.h3r[1] + q.h3r[2]);
+        }
+        buf ^= 1;
+        const int y = r - 4;  // the row whose sums are complete
+        if (y < y0) continue;
+        T F[2], M[2], GX[2], GY[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const WsPic<T> &q = pic[p];
+            T s9 = 0;
+#pragma unroll
+            for (int k = 0; k < 9; k++) s9 += q.h9[k];
+            const T s3 = q.s3c[1];  // rows: s3c[0] = y - 1, s3c[1] = y, s3c[2] = y + 1
+            F[p] = (T)9 * q.yv[0] - s3;
+            M[p] = (T)9 * s3 - s9;
+            GX[p] = (x > 0 && x < w - 1) ? q.s3r[0] - q.s3l[0] : (T)0;
+            GY[p] = (y > 0 && y < h - 1) ? q.s3c[2] - q.s3c[0] : (T)0;
+            if (mid[p] && inside) mid[p][(size_t)y * w + x] = (int32_t)M[p];
+        }
+        acc[0] += (A)F[0] * (A)F[0];
+        acc[1] += (A)M[0] * (A)M[0];
+        acc[2] += (A)F[1] * (A)F[1];
+        acc[3] += (A)M[1] * (A)M[1];
+        acc[4] += (A)F[0] * (A)F[1];
+        acc[5] += (A)M[0] * (A)M[1];
+        acc[6] += (A)GX[0] * (A)GX[0];
+        acc[7] += (A)GY[0] * (A)GY[0];
+        acc[8] += (A)GX[0] * (A)GY[0];
+        acc[9] += (A)GX[1] * (A)GX[1];
+        acc[10] += (A)GY[1] * (A)GY[1];
+        acc[11] += (A)GX[1] * (A)GY[1];
+        if (y % WIN == WIN - 1) {
+#pragma unroll
+            for (int k = 0; k < 12; k++) acc[k] = reduce_lanes(acc[k], WIN);
+            if (inside && (x % WIN) == 0) {
+                long long *o = P.sums + ((size_t)(y / WIN) * ww + x / WIN) * 12;
+#pragma unroll
+                for (int k = 0; k < 12; k++) o[k] = (long long)acc[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 12; k++) acc[k] = 0;
+        }
+    }
+}
+
+template <int WIN>
+static void launch_ws(const WsPair *pairs, int npairs, int max_w, int max_h, bool fp64, hipStream_t st)
+{
+    const int wcov = max_w / WIN * WIN, wh = max_h / WIN;
+    const dim3 grid((wcov + 255) / 256, (wh + kWsRows - 1) / kWsRows, npairs);
+    if (fp64) hipLaunchKernelGGL((window_stats_kernel<WIN, double, double>), grid, dim3(256), 0, st, pairs);
+    else hipLaunchKernelGGL((window_stats_kernel<WIN, int, long long>), grid, dim3(256), 0, st, pairs);
+}
+
+void launch_window_stats(const WsPair *pairs, int npairs, int max_w, int max_h, int win, bool fp64, hipStream_t st)
+{
+    if (npairs <= 0) return;
+    if (win == 16) launch_ws<16>(pairs, npairs, max_w, max_h, fp64, st);
+    else if (win == 32) launch_ws<32>(pairs, npairs, max_w, max_h, fp64, st);
+    else if (win == 64) launch_ws<64>(pairs, npairs, max_w, max_h, fp64, st);
+    else throw Error(1, "window size must be 16, 32 or 64");
+    hip_check(hipGetLastError(), "window stats");
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// Lasting texture.  One block of 256 threads per window; sums are exact (int64 or double on integers).
+// ---------------------------------------------------------------------------------------------------------------
+template <typename A, int N>
+__device__ __forceinline__ void block_sums(A (&v)[N], A (*part)[4])
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        v[k] = reduce_lanes(v[k], 64);
+        if (lane == 0) part[k][wave] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = part[k][0] + part[k][1] + part[k][2] + part[k][3];
+}
+
+// Plain form: the 13 source planes and the encode plane per (pair, centre), in double, 5 sums per window.
+template <int WIN>
+__global__ __launch_bounds__(256) void persist_plain_kernel(const PersistJob *jobs)
+{
+    __shared__ double part[5][4];
+    const PersistJob &J = jobs[blockIdx.z];
+    const int wx = blockIdx.x, wy = blockIdx.y, ww = J.w / WIN;
+    if (wx >= ww || wy >= J.h / WIN) return;
+    double v[5] = {0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < WIN * WIN; i += 256) {
+        const size_t k = (size_t)(wy * WIN + i / WIN) * J.w + wx * WIN + i % WIN;
+        const double a = (double)J.ring[0][k] + J.ring[2][k] + J.ring[4][k] + J.ring[8][k] + J.ring[10][k] + J.ring[12][k];
+        const double b = (double)J.ring[1][k] + J.ring[3][k] + J.ring[5][k] + J.ring[7][k] + J.ring[9][k] + J.ring[11][k];
+        const double t = a + b;
+        v[0] += (double)J.ring[6][k] * t;
+        v[1] += (double)J.mid_d[k] * t;
+        v[2] += a * b;
+        v[3] += a * a;
+        v[4] += b * b;
+    }
+    block_sums(v, part);
+    if (threadIdx.x == 0) {
+        long long *o = J.out + ((size_t)wy * ww + wx) * 5;
+        for (int k = 0; k < 5; k++) o[k] = (long long)v[k];
+    }
+}
+
+// Batched form, source side, once per (crop, centre): ST plane and [M_r(c) ST, SA SB, SA^2, SB^2] per window.
+template <int WIN>
+__global__ __launch_bounds__(256) void persist_src_kernel(const PersistJob *jobs)
+{
+    __shared__ long long part[4][4];
+    const PersistJob &J = jobs[blockIdx.z];
+    const int wx = blockIdx.x, wy = blockIdx.y, ww = J.w / WIN;
+    if (wx >= ww || wy >= J.h / WIN) return;
+    long long v[4] = {0, 0, 0, 0};
+    for (int i = threadIdx.x; i < WIN * WIN; i += 256) {
+        const size_t k = (size_t)(wy * WIN + i / WIN) * J.w + wx * WIN + i % WIN;
+        const int a = J.ring[0][k] + J.ring[2][k] + J.ring[4][k] + J.ring[8][k] + J.ring[10][k] + J.ring[12][k];
+        const int b = J.ring[1][k] + J.ring[3][k] + J.ring[5][k] + J.ring[7][k] + J.ring[9][k] + J.ring[11][k];
+        const int t = a + b;
+        J.st[k] = t;
+        v[0] += (long long)J.ring[6][k] * t;
+        v[1] += (long long)a * b;
+        v[2] += (long long)a * a;
+        v[3] += (long long)b * b;
+    }
+    block_sums(v, part);
+    if (threadIdx.x == 0) {
+        long long *o = J.out + ((size_t)wy * ww + wx) * 4;
+        for (int k = 0; k < 4; k++) o[k] = v[k];
+    }
+}
+
+// Batched form, encode side, per (encode, centre): [M_d(c) ST] per window.
+template <int WIN>
+__global__ __launch_bounds__(256) void persist_enc_kernel(const PersistJob *jobs)
+{
+    __shared__ long long part[1][4];
+    const PersistJob &J = jobs[blockIdx.z];
+    const int wx = blockIdx.x, wy = blockIdx.y, ww = J.w / WIN;
+    if (wx >= ww || wy >= J.h / WIN) return;
+    long long v[1] = {0};
+    for (int i = threadIdx.x; i < WIN * WIN; i += 256) {
+        const size_t k = (size_t)(wy * WIN + i / WIN) * J.w + wx * WIN + i % WIN;
+        v[0] += (long long)J.mid_d[k] * J.st[k];
+    }
+    block_sums(v, part);
+    if (threadIdx.x == 0) J.out[(size_t)wy * ww + wx] = v[0];
+}
+
+template <int WIN>
+static void launch_persist_t(const PersistJob *jobs, int njobs, int max_w, int max_h, int kind, hipStream_t st)
+{
+    const dim3 grid(max_w / WIN, max_h / WIN, njobs);
+    if (kind == 0) hipLaunchKernelGGL(persist_plain_kernel<WIN>, grid, dim3(256), 0, st, jobs);
+    else if (kind == 1) hipLaunchKernelGGL(persist_src_kernel<WIN>, grid, dim3(256), 0, st, jobs);
+    else hipLaunchKernelGGL(persist_enc_kernel<WIN>, grid, dim3(256), 0, st, jobs);
+}
+
+void launch_persist(const PersistJob *jobs, int njobs, int max_w, int max_h, int win, int kind, hipStream_t st)
+{
+    if (njobs <= 0) return;
+    if (win == 16) launch_persist_t<16>(jobs, njobs, max_w, max_h, kind, st);
+    else if (win == 32) launch_persist_t<32>(jobs, njobs, max_w, max_h, kind, st);
+    else if (win == 64) launch_persist_t<64>(jobs, njobs, max_w, max_h, kind, st);
+    else throw Error(1, "window size must be 16, 32 or 64");
+    hip_check(hipGetLastError(), "lasting texture");
+}
+
+}  // namespace es

@@ -1,0 +1,86 @@
+// Internal declarations shared by the HIP translation units of libencscore_hip: the device-side descriptors of
+// pictures and jobs, and the host-side launch wrappers each kernel file exports.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace es {
+
+constexpr int kScales = 5;
+constexpr int kDiffs = 4;
+constexpr uint16_t kNone = 0xFFFF;
+constexpr int kCvTileW = 64;     // c-value tiles are one wave wide
+constexpr int kMaxPad = 32;      // one wave covers the tile and both margins as 2 columns per lane: window <= 65
+constexpr int kMaxBand = 1024;   // band values, 10-bit codes
+constexpr int kHistBins = 20001; // per-window histograms: 0.05-point bins over [-500, 500]
+constexpr int kRing = 13, kRadius = 6;
+
+// Thrown by every host function on a HIP error or a bad argument; the C ABI turns it into a status code.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hip_check(hipError_t r, const char *what)
+{
+    if (r != hipSuccess) throw Error(2, std::string(what) + ": " + hipGetErrorString(r));
+}
+
+// CAMBI constants of one picture size (the reference's CambiParams less the per-scale size).
+struct CambiConsts {
+    int pad, band_base, band_size, vlt, mask_index, window;
+    int tvi[kDiffs], weight[kDiffs];
+};
+
+// A luma plane as the kernels read it: sample(x, y) = plane[y * pitch + x] << shift, `bytes` 1 or 2 per sample, and
+// with from8 the 8-bit anti-dithering mean of cambi.c applied after the shift.
+struct PlaneIn {
+    const void *ptr;
+    int pitch, bytes, shift, from8;
+};
+
+// One picture for the CAMBI kernels.  The packed scale planes hold value | mask << 15 per pixel, scale s at
+// planes + off[s]; the optional full c-value planes and the compact lists of positive c-values use the same offsets.
+struct CambiPic {
+    PlaneIn in;
+    int sw[kScales], sh[kScales];
+    long long off[kScales];
+    uint16_t *planes;
+    float *cvals;                 // full c-value planes or nullptr
+    float *list;                  // positive c-values, scale s from list + off[s]
+    unsigned long long *sums;     // [kScales] exact sums, units of 2^-24
+    unsigned int *counts;         // [kScales] positive c-values (the list lengths)
+    const CambiConsts *consts;    // device pointer
+    const float *lut;             // reciprocal table
+};
+
+// Radix-select state of one (picture, scale) top-k: the k-th largest positive c-value when more than k are positive.
+struct TopkJob {
+    const float *list;
+    const unsigned int *count;
+    unsigned long long k;
+    unsigned int *hist;            // [512] counts of the current digit
+    unsigned long long *hsum;      // [512] exact sums per digit
+    unsigned int *ticket;          // blocks done in the current pass
+    unsigned int prefix, remaining;
+    unsigned long long above, above_fx;
+};
+
+// Host-side launch wrappers (cambi_kernels.hip).
+void launch_cambi_base(const CambiPic *pics, const int *tile_first, int npics, int total_tiles, hipStream_t st);
+void launch_cambi_scale(const CambiPic *pics, const long long *px_first, int npics, long long total_px, int s,
+                        hipStream_t st);
+// tile_first[nslots] (device) holds the total; total_tiles sizes the grid when not persistent.
+void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
+                          int persistent, unsigned int *work_counter, int blocks, hipStream_t st);
+void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks_per_job, hipStream_t st);
+void launch_cambi_finish(const CambiPic *pics, TopkJob *jobs, int npics, double *scores, hipStream_t st);
+void launch_clear(unsigned long long *a, long long na, unsigned int *b, long long nb, hipStream_t st);
+int base_tiles(int w, int h);
+int cvalue_tiles(int w, int h, int tile_h);
+
+}  // namespace es

@@ -1,0 +1,776 @@
+// es_scorer: the measure path of one source against its encodes, frame step by frame step (src/main.cpp:516-643).
+//
+// Plain form: the reference's structure on the GPU.  Per frame, one CAMBI instance per picture (two per file,
+// alternating by frame parity), one window-stats and one lasting-texture launch per encode in double precision, and
+// the collect on the host, overlapped with the GPU's work on the next frame (src/main.cpp:666-687).
+//
+// Batched form: `batch` frame steps per round.  Every kernel covers all pictures or pairs of the round in one launch
+// (CAMBI's c-values as a persistent kernel pulling tiles from a device counter), the energies in exact integers, the
+// source side of the lasting texture once per crop, and the collect and selects on the device; the round is captured
+// once as a HIP graph per staging buffer and replayed.  Per-pair crops and uneven encode lengths are handled by
+// descriptors: an encode that has ended leaves its pictures and jobs inactive.
+#include "cambi_host.hpp"
+#include "energy.hpp"
+#include "host_common.hpp"
+#include "stats_host.hpp"
+
+#include <algorithm>
+#include <memory>
+
+namespace es {
+
+struct Crop {
+    int w, h, cx, cy, windows, ww, wh;
+    long len;  // frames while any of its encodes runs
+    CambiInfo ci;
+};
+
+struct EncodeState {
+    es_encode_desc d;
+    int crop, windows;
+    std::vector<Accum> seg;
+    std::vector<double> cambi, cambi_src, quantile;
+};
+
+class ScorerBase {
+public:
+    explicit ScorerBase(const es_scorer_config &c) : cfg(c)
+    {
+        require(c.n_encodes >= 1 && c.encodes, "at least one encode");
+        require(c.src_bytes == 1 || c.src_bytes == 2, "samples must be 1 or 2 bytes");
+        require(c.depth == 8 || c.depth == 10, "depth must be 8 or 10");
+        require(c.win == 16 || c.win == 32 || c.win == 64, "window size must be 16, 32 or 64");
+        require(c.per_segment >= 1, "per_segment must be positive");
+        require(c.src_pitch >= c.src_w, "source pitch below its width");
+        window_base = c.window_base > 0 ? c.window_base : 65;
+        for (int e = 0; e < c.n_encodes; e++) {
+            const es_encode_desc &d = c.encodes[e];
+            require(d.w >= c.win && d.h >= c.win && d.w <= c.src_w && d.h <= c.src_h, "encode size must fit the source");
+            require(d.length >= 0 && d.pitch >= d.w && (d.bytes == 1 || d.bytes == 2), "bad encode description");
+            EncodeState s;
+            s.d = d;
+            s.crop = -1;
+            for (size_t k = 0; k < crops.size(); k++)
+                if (crops[k].w == d.w && crops[k].h == d.h) s.crop = (int)k;
+            if (s.crop < 0) {
+                Crop cr{};
+                cr.w = d.w;
+                cr.h = d.h;
+                cr.cx = ((c.src_w - d.w) / 2) & ~1;  // centred, even (src/main.cpp:466-467)
+                cr.cy = ((c.src_h - d.h) / 2) & ~1;
+                cr.ww = d.w / c.win;
+                cr.wh = d.h / c.win;
+                cr.windows = cr.ww * cr.wh;
+                cr.len = 0;
+                cr.ci = cambi_info(d.w, d.h, c.depth, window_base);
+                crops.push_back(cr);
+                s.crop = (int)crops.size() - 1;
+            }
+            Crop &cr = crops[s.crop];
+            cr.len = std::max(cr.len, (long)d.length);
+            s.windows = cr.windows;
+            s.seg.resize(d.length ? (size_t)((d.length + c.per_segment - 1) / c.per_segment) : 0);
+            enc.push_back(std::move(s));
+        }
+        encodes_copy.assign(c.encodes, c.encodes + c.n_encodes);
+        cfg.encodes = encodes_copy.data();
+        up = 10 - c.depth;
+    }
+    virtual ~ScorerBase() = default;
+    virtual void submit(int n, const void *const *src, const void *const *encp, hipStream_t st) = 0;
+    virtual void sync() = 0;
+    virtual void segment_hist(int e, int seg, Accum &a) { (void)e, (void)seg, (void)a; }
+
+    es_scorer_config cfg;
+    std::vector<es_encode_desc> encodes_copy;
+    std::vector<Crop> crops;
+    std::vector<EncodeState> enc;
+    int window_base, up;
+    long frames = 0;  // frame steps submitted
+
+protected:
+    bool active(int e, long f) const { return f < enc[e].d.length; }
+    bool crop_active(int c, long f) const
+    {
+        for (size_t e = 0; e < enc.size(); e++)
+            if (enc[e].crop == c && active((int)e, f)) return true;
+        return false;
+    }
+    PlaneIn src_plane(const void *src, int c) const
+    {
+        const Crop &cr = crops[c];
+        const char *p = (const char *)src + ((size_t)cr.cy * cfg.src_pitch + cr.cx) * cfg.src_bytes;
+        return PlaneIn{p, cfg.src_pitch, cfg.src_bytes, cfg.src_shift, 0};
+    }
+    PlaneIn enc_plane(const void *p, int e) const
+    {
+        const es_encode_desc &d = enc[e].d;
+        return PlaneIn{p, d.pitch, d.bytes, d.shift, 0};
+    }
+    Accum &segment(int e, long f) { return enc[e].seg[(size_t)(f / cfg.per_segment)]; }
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// Plain form
+// ---------------------------------------------------------------------------------------------------------------
+class PlainScorer final : public ScorerBase {
+public:
+    explicit PlainScorer(const es_scorer_config &c) : ScorerBase(c)
+    {
+        for (const Crop &cr : crops) {
+            CropBufs b;
+            for (auto &u : b.cambi) u = std::make_unique<CambiUnit>(cr.w, cr.h, c.depth, window_base, false, c.tile_h);
+            b.ring.alloc((size_t)kRing * cr.w * cr.h * 4);
+            hip_check(hipMemset(b.ring.p, 0, b.ring.bytes), "clearing");
+            cb.push_back(std::move(b));
+        }
+        for (size_t e = 0; e < enc.size(); e++) {
+            const Crop &cr = crops[enc[e].crop];
+            EncBufs b;
+            for (auto &u : b.cambi) u = std::make_unique<CambiUnit>(cr.w, cr.h, c.depth, window_base, false, c.tile_h);
+            b.mid.alloc((size_t)(kRadius + 1) * cr.w * cr.h * 4);
+            hip_check(hipMemset(b.mid.p, 0, b.mid.bytes), "clearing");
+            for (int k = 0; k < 2; k++) {
+                b.sums[k].alloc((size_t)cr.windows * 12 * 8);
+                b.psums[k].alloc((size_t)cr.windows * 5 * 8);
+                b.hsums[k].alloc((size_t)cr.windows * 12 * 8);
+                b.hpsums[k].alloc((size_t)cr.windows * 5 * 8);
+            }
+            b.masks.assign(kRadius + 1, std::vector<uint8_t>(cr.windows, 0));
+            eb.push_back(std::move(b));
+        }
+        for (auto &e : done) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "creating an event");
+    }
+    ~PlainScorer() override
+    {
+        for (auto &e : done)
+            if (e) (void)hipEventDestroy(e);
+    }
+
+    void submit(int n, const void *const *src, const void *const *encp, hipStream_t st) override
+    {
+        for (int i = 0; i < n; i++) {
+            queue(frames, src[i], encp + (size_t)i * enc.size(), st);
+            if (frames > 0) collect(frames - 1);
+            frames++;
+        }
+        pending = frames > 0;
+    }
+
+    void sync() override
+    {
+        if (pending && frames > 0) collect(frames - 1);
+        pending = false;
+    }
+
+private:
+    struct CropBufs {
+        std::unique_ptr<CambiUnit> cambi[2];
+        DevBuf ring;  // 13 mid planes by frame % 13
+    };
+    struct EncBufs {
+        std::unique_ptr<CambiUnit> cambi[2];
+        DevBuf mid;  // 7 mid planes by frame % 7
+        DevBuf sums[2], psums[2];
+        HostBuf hsums[2], hpsums[2];
+        std::vector<std::vector<uint8_t>> masks;
+    };
+    std::vector<CropBufs> cb;
+    std::vector<EncBufs> eb;
+    hipEvent_t done[2] = {};
+    bool pending = false;
+    long collected = -1;
+
+    int32_t *ring_plane(int c, long f) { return cb[c].ring.as<int32_t>() + (size_t)(f % kRing) * crops[c].w * crops[c].h; }
+    int32_t *mid_plane(int e, long f)
+    {
+        const Crop &cr = crops[enc[e].crop];
+        return eb[e].mid.as<int32_t>() + (size_t)(f % (kRadius + 1)) * cr.w * cr.h;
+    }
+
+    // queue (src/main.cpp:516-574): the GPU work of frame f
+    void queue(long f, const void *src, const void *const *encp, hipStream_t st)
+    {
+        const int at = (int)(f % 2);
+        for (size_t c = 0; c < crops.size(); c++) {
+            if (!crop_active((int)c, f)) continue;
+            const PlaneIn p = src_plane(src, (int)c);
+            cb[c].cambi[at]->launch(p.ptr, p.pitch, p.bytes, p.shift, st);
+        }
+        std::vector<bool> wrote(crops.size(), false);
+        for (size_t e = 0; e < enc.size(); e++) {
+            if (!active((int)e, f)) continue;
+            const int c = enc[e].crop;
+            const Crop &cr = crops[c];
+            const PlaneIn pe = enc_plane(encp[e], (int)e);
+            eb[e].cambi[at]->launch(pe.ptr, pe.pitch, pe.bytes, pe.shift, st);
+            WsPair pr{};
+            pr.r = src_plane(src, c);
+            pr.d = pe;
+            pr.w = cr.w;
+            pr.h = cr.h;
+            pr.up = up;
+            pr.mid_r = wrote[c] ? nullptr : ring_plane(c, f);
+            wrote[c] = true;
+            pr.mid_d = mid_plane((int)e, f);
+            pr.sums = eb[e].sums[at].as<long long>();
+            const auto *dp = (const WsPair *)arena().stage(&pr, sizeof pr, st);
+            launch_window_stats(dp, 1, cr.w, cr.h, cfg.win, true, st);
+            arena().done(st);
+            hip_check(hipMemcpyAsync(eb[e].hsums[at].p, eb[e].sums[at].p, eb[e].sums[at].bytes, hipMemcpyDeviceToHost, st), "reading figures");
+            if (f >= 2 * kRadius) {
+                const long centre = f - kRadius;
+                PersistJob j{};
+                for (int o = -kRadius; o <= kRadius; o++) j.ring[o + kRadius] = ring_plane(c, centre + o);
+                j.mid_d = mid_plane((int)e, centre);
+                j.w = cr.w;
+                j.h = cr.h;
+                j.out = eb[e].psums[at].as<long long>();
+                const auto *dj = (const PersistJob *)arena().stage(&j, sizeof j, st);
+                launch_persist(dj, 1, cr.w, cr.h, cfg.win, 0, st);
+                arena().done(st);
+                hip_check(hipMemcpyAsync(eb[e].hpsums[at].p, eb[e].psums[at].p, eb[e].psums[at].bytes, hipMemcpyDeviceToHost, st), "reading figures");
+            }
+        }
+        hip_check(hipEventRecord(done[at], st), "event");
+    }
+
+    // collect (src/main.cpp:576-643): the figures of frame f into the segments
+    void collect(long f)
+    {
+        if (f <= collected) return;
+        collected = f;
+        const int at = (int)(f % 2);
+        hip_check(hipEventSynchronize(done[at]), "waiting for a frame");
+        std::vector<double> src_score(crops.size(), 0.0);
+        for (size_t c = 0; c < crops.size(); c++)
+            if (crop_active((int)c, f)) src_score[c] = cb[c].cambi[at]->finish();
+        for (size_t e = 0; e < enc.size(); e++) {
+            if (!active((int)e, f)) continue;
+            EncodeState &s = enc[e];
+            const double score = eb[e].cambi[at]->finish();
+            s.cambi.push_back(score);
+            s.cambi_src.push_back(src_score[s.crop]);
+            long long ti[12];
+            double to[4];
+            Accum &acc = segment((int)e, f);
+            const double q = collect_frame_host(acc, eb[e].hsums[at].as<long long>(), s.windows,
+                                                eb[e].masks[f % (kRadius + 1)].data(), ti, to);
+            s.quantile.push_back(q);
+            acc.add_frame(score, src_score[s.crop], ti, to);
+            if (f >= 2 * kRadius) {
+                const long centre = f - kRadius;
+                long long tl[3];
+                Accum &cacc = segment((int)e, centre);
+                collect_lasting_host(cacc, eb[e].hpsums[at].as<long long>(), eb[e].masks[centre % (kRadius + 1)].data(),
+                                     s.windows, tl);
+                cacc.add_lasting(tl);
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------------------------
+// Batched form
+// ---------------------------------------------------------------------------------------------------------------
+class BatchedScorer final : public ScorerBase {
+public:
+    explicit BatchedScorer(const es_scorer_config &c) : ScorerBase(c)
+    {
+        B = c.batch > 0 ? c.batch : 16;
+        require(B <= 64, "batch must be at most 64 frames");
+        R = B + 16;
+        tile_h = c.tile_h > 0 ? c.tile_h : 32;
+        const int C = (int)crops.size(), E = (int)enc.size();
+        roles = C + E;
+        P = B * roles;
+        // CAMBI slots: one per (frame of the round, role); roles 0..C-1 are the source's crops, then the encodes
+        consts.alloc((size_t)C * sizeof(CambiConsts));
+        std::vector<CambiConsts> hk(C);
+        for (int k = 0; k < C; k++) {
+            const CambiInfo &ci = crops[k].ci;
+            CambiConsts &K = hk[k];
+            K.pad = ci.pad;
+            K.band_base = ci.band_base;
+            K.band_size = ci.band_size;
+            K.vlt = ci.vlt;
+            K.mask_index = ci.mask_index;
+            K.window = ci.window;
+            for (int d = 0; d < kDiffs; d++) {
+                K.tvi[d] = ci.tvi[d];
+                K.weight[d] = ci.weight[d];
+            }
+        }
+        hip_check(hipMemcpy(consts.p, hk.data(), consts.bytes, hipMemcpyHostToDevice), "cambi constants");
+        slot_off.resize(P + 1);
+        slot_off[0] = 0;
+        for (int p = 0; p < P; p++) slot_off[p + 1] = slot_off[p] + (size_t)crops[crop_of_role(p % roles)].ci.off[kScales];
+        planes.alloc(slot_off[P] * 2);
+        list.alloc(slot_off[P] * 4);
+        sums.alloc((size_t)P * kScales * 8);
+        counts.alloc((size_t)P * kScales * 4 + 64);  // + the work counter
+        thist.alloc((size_t)P * kScales * 512 * 4);
+        thsum.alloc((size_t)P * kScales * 512 * 8);
+        tticket.alloc((size_t)P * kScales * 4);
+        hip_check(hipMemset(thist.p, 0, thist.bytes), "clearing");
+        hip_check(hipMemset(thsum.p, 0, thsum.bytes), "clearing");
+        hip_check(hipMemset(tticket.p, 0, tticket.bytes), "clearing");
+        std::vector<TopkJob> jobs((size_t)P * kScales);
+        for (int p = 0; p < P; p++) {
+            const CambiInfo &ci = crops[crop_of_role(p % roles)].ci;
+            for (int s = 0; s < kScales; s++) {
+                TopkJob &j = jobs[(size_t)p * kScales + s];
+                j = TopkJob{};
+                j.list = list.as<float>() + slot_off[p] + ci.off[s];
+                j.count = counts.as<unsigned>() + (size_t)p * kScales + s;
+                j.k = (unsigned long long)ci.k[s];
+                j.hist = thist.as<unsigned>() + ((size_t)p * kScales + s) * 512;
+                j.hsum = thsum.as<unsigned long long>() + ((size_t)p * kScales + s) * 512;
+                j.ticket = tticket.as<unsigned>() + (size_t)p * kScales + s;
+            }
+        }
+        topk.alloc(jobs.size() * sizeof(TopkJob));
+        hip_check(hipMemcpy(topk.p, jobs.data(), topk.bytes, hipMemcpyHostToDevice), "top-k jobs");
+        scores.alloc((size_t)P * 8);
+        max_base_tiles = 0;
+        max_px.assign(kScales, 0);
+        for (const Crop &cr : crops) {
+            max_base_tiles = std::max(max_base_tiles, base_tiles(cr.w, cr.h));
+            for (int s = 0; s < kScales; s++) max_px[s] = std::max(max_px[s], cr.ci.n[s]);
+            max_w = std::max(max_w, cr.w);
+            max_h = std::max(max_h, cr.h);
+            max_windows = std::max(max_windows, cr.windows);
+        }
+        // energies, lasting texture, collect
+        for (int k = 0; k < C; k++) {
+            CropDev d;
+            const size_t n = (size_t)crops[k].w * crops[k].h;
+            d.ring.alloc((size_t)R * n * 4);
+            hip_check(hipMemset(d.ring.p, 0, d.ring.bytes), "clearing");
+            d.st.alloc((size_t)B * n * 4);
+            d.psrc.alloc((size_t)B * crops[k].windows * 4 * 8);
+            cd.push_back(std::move(d));
+        }
+        for (int e = 0; e < E; e++) {
+            EncDev d;
+            const Crop &cr = crops[enc[e].crop];
+            const size_t n = (size_t)cr.w * cr.h;
+            d.mid.alloc((size_t)R * n * 4);
+            hip_check(hipMemset(d.mid.p, 0, d.mid.bytes), "clearing");
+            d.masks.alloc((size_t)R * cr.windows);
+            hip_check(hipMemset(d.masks.p, 0, d.masks.bytes), "clearing");
+            d.sums.alloc((size_t)B * cr.windows * 12 * 8);
+            d.penc.alloc((size_t)B * cr.windows * 8);
+            d.hist.alloc(std::max<size_t>(1, enc[e].seg.size()) * 6 * kHistBins * 8);
+            hip_check(hipMemset(d.hist.p, 0, d.hist.bytes), "clearing");
+            ed.push_back(std::move(d));
+        }
+        tot_int.alloc((size_t)B * E * 12 * 8);
+        tot_or.alloc((size_t)B * E * 4 * 8);
+        tot_last.alloc((size_t)B * E * 3 * 8);
+        quant.alloc((size_t)B * E * 8);
+        scratch.alloc((size_t)B * E * max_windows * 8);
+        // descriptors: device arrays, staged from pinned host memory per round (two staging buffers)
+        desc_bytes = align(P * sizeof(CambiPic)) + align(((size_t)P * kScales + 1) * 4) + align((size_t)B * E * sizeof(WsPair)) +
+                     align((size_t)B * C * sizeof(PersistJob)) + align((size_t)B * E * sizeof(PersistJob)) +
+                     align((size_t)B * E * sizeof(CollectJob));
+        desc.alloc(desc_bytes);
+        res_bytes = align((size_t)P * 8) + align((size_t)B * E * 12 * 8) + align((size_t)B * E * 4 * 8) +
+                    align((size_t)B * E * 3 * 8) + align((size_t)B * E * 8);
+        for (int k = 0; k < 2; k++) {
+            stage[k].alloc(desc_bytes);
+            res[k].alloc(res_bytes);
+            hip_check(hipEventCreateWithFlags(&done[k], hipEventDisableTiming), "creating an event");
+        }
+        hip_check(hipStreamCreateWithFlags(&work, hipStreamNonBlocking), "creating a stream");
+        hip_check(hipEventCreateWithFlags(&joined, hipEventDisableTiming), "creating an event");
+        use_graph = c.use_graph != 0;
+    }
+
+    ~BatchedScorer() override
+    {
+        for (int k = 0; k < 2; k++) {
+            if (done[k]) {
+                (void)hipEventSynchronize(done[k]);
+                (void)hipEventDestroy(done[k]);
+            }
+            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+            if (graph[k]) (void)hipGraphDestroy(graph[k]);
+        }
+        if (work) {
+            (void)hipStreamSynchronize(work);
+            (void)hipStreamDestroy(work);
+        }
+        if (joined) (void)hipEventDestroy(joined);
+    }
+
+    // The rounds run on the scorer's own stream (a graph cannot be captured on the null stream), ordered after the
+    // caller's earlier work on `st`, and the caller's later work on `st` waits for them.
+    void submit(int n, const void *const *src, const void *const *encp, hipStream_t st) override
+    {
+        hip_check(hipEventRecord(joined, st), "event");
+        hip_check(hipStreamWaitEvent(work, joined, 0), "event");
+        for (int i = 0; i < n; i += B) {
+            const int m = std::min(B, n - i);
+            round(m, src + i, encp + (size_t)i * enc.size(), work);
+        }
+        hip_check(hipEventRecord(joined, work), "event");
+        hip_check(hipStreamWaitEvent(st, joined, 0), "event");
+    }
+
+    void sync() override
+    {
+        // fold the outstanding rounds in order
+        for (int k = 0; k < 2; k++) {
+            const int p = (int)((rounds + k) % 2);
+            if (pend[p].n) fold(p);
+        }
+    }
+
+    void segment_hist(int e, int seg, Accum &a) override
+    {
+        hip_check(hipMemcpy(a.hist.data(), ed[e].hist.as<unsigned long long>() + (size_t)seg * 6 * kHistBins,
+                            (size_t)6 * kHistBins * 8, hipMemcpyDeviceToHost),
+                  "reading histograms");
+    }
+
+private:
+    static size_t align(size_t n) { return (n + 255) & ~(size_t)255; }
+    int crop_of_role(int r) const { return r < (int)crops.size() ? r : enc[r - crops.size()].crop; }
+
+    struct CropDev {
+        DevBuf ring, st, psrc;
+    };
+    struct EncDev {
+        DevBuf mid, masks, sums, penc, hist;
+    };
+    struct Pending {
+        long f0 = 0;
+        int n = 0;
+    };
+
+    int B, R, P, roles, tile_h, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
+    std::vector<long long> max_px;
+    std::vector<size_t> slot_off;
+    DevBuf consts, planes, list, sums, counts, thist, thsum, tticket, topk, scores;
+    std::vector<CropDev> cd;
+    std::vector<EncDev> ed;
+    DevBuf tot_int, tot_or, tot_last, quant, scratch, desc;
+    size_t desc_bytes, res_bytes;
+    HostBuf stage[2], res[2];
+    hipEvent_t done[2] = {}, joined = nullptr;
+    hipStream_t work = nullptr;
+    hipGraph_t graph[2] = {};
+    hipGraphExec_t exec[2] = {};
+    Pending pend[2];
+    long rounds = 0;
+    bool use_graph = false;
+
+    template <class T>
+    T *dptr(size_t off) const { return (T *)(desc.as<char>() + off); }
+
+    // offsets of the descriptor arrays
+    size_t o_pics() const { return 0; }
+    size_t o_first() const { return align((size_t)P * sizeof(CambiPic)); }
+    size_t o_ws() const { return o_first() + align(((size_t)P * kScales + 1) * 4); }
+    size_t o_psrc() const { return o_ws() + align((size_t)B * enc.size() * sizeof(WsPair)); }
+    size_t o_penc() const { return o_psrc() + align((size_t)B * crops.size() * sizeof(PersistJob)); }
+    size_t o_col() const { return o_penc() + align((size_t)B * enc.size() * sizeof(PersistJob)); }
+    size_t r_scores() const { return 0; }
+    size_t r_ti() const { return align((size_t)P * 8); }
+    size_t r_to() const { return r_ti() + align((size_t)B * enc.size() * 12 * 8); }
+    size_t r_tl() const { return r_to() + align((size_t)B * enc.size() * 4 * 8); }
+    size_t r_q() const { return r_tl() + align((size_t)B * enc.size() * 3 * 8); }
+
+    void fill(int p, long f0, int n, const void *const *src, const void *const *encp)
+    {
+        char *h = stage[p].as<char>();
+        std::memset(h, 0, desc_bytes);
+        const int C = (int)crops.size(), E = (int)enc.size();
+        CambiPic *pics = (CambiPic *)(h + o_pics());
+        int *first = (int *)(h + o_first());
+        WsPair *ws = (WsPair *)(h + o_ws());
+        PersistJob *psj = (PersistJob *)(h + o_psrc());
+        PersistJob *pej = (PersistJob *)(h + o_penc());
+        CollectJob *cj = (CollectJob *)(h + o_col());
+        int t = 0;
+        for (int i = 0; i < B; i++) {
+            const long f = f0 + i;
+            for (int r = 0; r < roles; r++) {
+                const int pidx = i * roles + r;
+                CambiPic &pc = pics[pidx];
+                const int c = crop_of_role(r);
+                const CambiInfo &ci = crops[c].ci;
+                const bool on = i < n && (r < C ? crop_active(r, f) : active(r - C, f));
+                for (int s = 0; s < kScales; s++) {
+                    first[pidx * kScales + s] = t;
+                    if (on) t += cvalue_tiles(ci.sw[s], ci.sh[s], tile_h);
+                    pc.sw[s] = on ? ci.sw[s] : 0;
+                    pc.sh[s] = on ? ci.sh[s] : 0;
+                    pc.off[s] = ci.off[s];
+                }
+                if (on) {
+                    pc.in = r < C ? src_plane(src[i], r) : enc_plane(encp[(size_t)i * E + (r - C)], r - C);
+                    if (cfg.depth == 8) {
+                        pc.in.shift += 2;
+                        pc.in.from8 = 1;
+                    }
+                }
+                pc.planes = planes.as<uint16_t>() + slot_off[pidx];
+                pc.cvals = nullptr;
+                pc.list = list.as<float>() + slot_off[pidx];
+                pc.sums = sums.as<unsigned long long>() + (size_t)pidx * kScales;
+                pc.counts = counts.as<unsigned>() + (size_t)pidx * kScales;
+                pc.consts = consts.as<CambiConsts>() + c;
+                pc.lut = device_lut(ci.lut_size);
+            }
+        }
+        first[P * kScales] = t;
+        for (int i = 0; i < B; i++) {
+            const long f = f0 + i;
+            std::vector<bool> wrote(C, false);
+            for (int e = 0; e < E; e++) {
+                const int c = enc[e].crop;
+                const Crop &cr = crops[c];
+                const size_t np = (size_t)cr.w * cr.h;
+                const bool on = i < n && active(e, f);
+                WsPair &w = ws[(size_t)i * E + e];
+                CollectJob &j = cj[(size_t)i * E + e];
+                PersistJob &pe = pej[(size_t)i * E + e];
+                if (!on) continue;
+                w.r = src_plane(src[i], c);
+                w.d = enc_plane(encp[(size_t)i * E + e], e);
+                w.w = cr.w;
+                w.h = cr.h;
+                w.up = up;
+                w.mid_r = wrote[c] ? nullptr : cd[c].ring.as<int32_t>() + (size_t)(f % R) * np;
+                wrote[c] = true;
+                w.mid_d = ed[e].mid.as<int32_t>() + (size_t)(f % R) * np;
+                w.sums = ed[e].sums.as<long long>() + (size_t)i * cr.windows * 12;
+                j.sums = w.sums;
+                j.windows = cr.windows;
+                j.mask = ed[e].masks.as<uint8_t>() + (size_t)(f % R) * cr.windows;
+                j.hist = ed[e].hist.as<unsigned long long>() + (size_t)(f / cfg.per_segment) * 6 * kHistBins;
+                j.tot_int = tot_int.as<long long>() + ((size_t)i * E + e) * 12;
+                j.tot_orient = tot_or.as<double>() + ((size_t)i * E + e) * 4;
+                j.quantile = quant.as<double>() + (size_t)i * E + e;
+                j.tot_lasting = tot_last.as<long long>() + ((size_t)i * E + e) * 3;
+                if (f >= 2 * kRadius) {
+                    const long centre = f - kRadius;
+                    pe.mid_d = ed[e].mid.as<int32_t>() + (size_t)(centre % R) * np;
+                    pe.st = cd[c].st.as<int32_t>() + (size_t)i * np;
+                    pe.w = cr.w;
+                    pe.h = cr.h;
+                    pe.out = ed[e].penc.as<long long>() + (size_t)i * cr.windows;
+                    j.psrc = cd[c].psrc.as<long long>() + (size_t)i * cr.windows * 4;
+                    j.penc = pe.out;
+                    j.cmask = ed[e].masks.as<uint8_t>() + (size_t)(centre % R) * cr.windows;
+                    j.lhist = ed[e].hist.as<unsigned long long>() + (size_t)(centre / cfg.per_segment) * 6 * kHistBins;
+                    j.lasting_plain = 0;
+                }
+            }
+            for (int c = 0; c < C; c++) {
+                PersistJob &ps = psj[(size_t)i * C + c];
+                if (!(i < n && f >= 2 * kRadius && crop_active(c, f))) continue;
+                const Crop &cr = crops[c];
+                const size_t np = (size_t)cr.w * cr.h;
+                const long centre = f - kRadius;
+                for (int o = -kRadius; o <= kRadius; o++) ps.ring[o + kRadius] = cd[c].ring.as<int32_t>() + (size_t)((centre + o) % R) * np;
+                ps.st = cd[c].st.as<int32_t>() + (size_t)i * np;
+                ps.w = cr.w;
+                ps.h = cr.h;
+                ps.out = cd[c].psrc.as<long long>() + (size_t)i * cr.windows * 4;
+            }
+        }
+    }
+
+    // The round's pipeline on `st`: the descriptor copy, the kernels and the copy of the results.
+    void pipeline(int p, hipStream_t st)
+    {
+        const int C = (int)crops.size(), E = (int)enc.size();
+        hip_check(hipMemcpyAsync(desc.p, stage[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
+        const CambiPic *pics = dptr<CambiPic>(o_pics());
+        const int *first = dptr<int>(o_first());
+        unsigned *counter = counts.as<unsigned>() + (size_t)P * kScales;
+        launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
+        launch_cambi_base(pics, nullptr, P, max_base_tiles, st);
+        for (int s = 1; s < kScales; s++) launch_cambi_scale(pics, nullptr, P, max_px[s], s, st);
+        launch_cambi_cvalues(pics, first, P * kScales, 0, tile_h, 1, counter, 2048, st);
+        for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, 16, st);
+        launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st);
+        launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st);
+        launch_persist(dptr<PersistJob>(o_psrc()), B * C, max_w, max_h, cfg.win, 1, st);
+        launch_persist(dptr<PersistJob>(o_penc()), B * E, max_w, max_h, cfg.win, 2, st);
+        launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 0, st);
+        launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 1, st);
+        char *r = res[p].as<char>();
+        hip_check(hipMemcpyAsync(r + r_scores(), scores.p, (size_t)P * 8, hipMemcpyDeviceToHost, st), "results");
+        hip_check(hipMemcpyAsync(r + r_ti(), tot_int.p, tot_int.bytes, hipMemcpyDeviceToHost, st), "results");
+        hip_check(hipMemcpyAsync(r + r_to(), tot_or.p, tot_or.bytes, hipMemcpyDeviceToHost, st), "results");
+        hip_check(hipMemcpyAsync(r + r_tl(), tot_last.p, tot_last.bytes, hipMemcpyDeviceToHost, st), "results");
+        hip_check(hipMemcpyAsync(r + r_q(), quant.p, quant.bytes, hipMemcpyDeviceToHost, st), "results");
+    }
+
+    void round(int n, const void *const *src, const void *const *encp, hipStream_t st)
+    {
+        const int p = (int)(rounds % 2);
+        if (pend[p].n) fold(p);
+        fill(p, frames, n, src, encp);
+        if (use_graph) {
+            if (!exec[p]) {
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+                pipeline(p, st);
+                hip_check(hipStreamEndCapture(st, &graph[p]), "graph capture");
+                hip_check(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "graph instantiate");
+            }
+            hip_check(hipGraphLaunch(exec[p], st), "graph launch");
+        } else {
+            pipeline(p, st);
+        }
+        hip_check(hipEventRecord(done[p], st), "event");
+        pend[p].f0 = frames;
+        pend[p].n = n;
+        frames += n;
+        rounds++;
+    }
+
+    void fold(int p)
+    {
+        hip_check(hipEventSynchronize(done[p]), "waiting for a round");
+        const char *r = res[p].as<char>();
+        const double *sc = (const double *)(r + r_scores());
+        const long long *ti = (const long long *)(r + r_ti());
+        const double *to = (const double *)(r + r_to());
+        const long long *tl = (const long long *)(r + r_tl());
+        const double *q = (const double *)(r + r_q());
+        const int C = (int)crops.size(), E = (int)enc.size();
+        for (int i = 0; i < pend[p].n; i++) {
+            const long f = pend[p].f0 + i;
+            for (int e = 0; e < E; e++) {
+                if (!active(e, f)) continue;
+                EncodeState &s = enc[e];
+                const size_t k = (size_t)i * E + e;
+                const double score = sc[i * roles + C + e], src = sc[i * roles + s.crop];
+                s.cambi.push_back(score);
+                s.cambi_src.push_back(src);
+                s.quantile.push_back(q[k]);
+                segment(e, f).add_frame(score, src, ti + k * 12, to + k * 4);
+                if (f >= 2 * kRadius) segment(e, f - kRadius).add_lasting(tl + k * 3);
+            }
+        }
+        pend[p].n = 0;
+    }
+};
+
+}  // namespace es
+
+using namespace es;
+
+struct es_scorer_s {
+    std::unique_ptr<ScorerBase> impl;
+};
+
+extern "C" {
+
+int es_scorer_create(const es_scorer_config *cfg, es_scorer **out)
+{
+    return guard([&] {
+        ensure_init();
+        require(cfg && out, "null argument");
+        auto s = std::make_unique<es_scorer_s>();
+        if (cfg->mode == ES_PLAIN) s->impl = std::make_unique<PlainScorer>(*cfg);
+        else if (cfg->mode == ES_BATCHED) s->impl = std::make_unique<BatchedScorer>(*cfg);
+        else throw Error(ES_EINVAL, "mode must be ES_PLAIN or ES_BATCHED");
+        *out = s.release();
+    });
+}
+
+int es_scorer_destroy(es_scorer *s)
+{
+    return guard([&] { delete s; });
+}
+
+int es_scorer_submit(es_scorer *s, int nframes, const void *const *src, const void *const *enc, es_stream stream)
+{
+    return guard([&] {
+        require(s && src && enc && nframes >= 0, "bad argument");
+        ensure_init();
+        s->impl->submit(nframes, src, enc, hs(stream));
+    });
+}
+
+int es_scorer_sync(es_scorer *s)
+{
+    return guard([&] {
+        require(s != nullptr, "null scorer");
+        s->impl->sync();
+    });
+}
+
+int es_scorer_frames(es_scorer *s, long *frames)
+{
+    return guard([&] {
+        require(s && frames, "null argument");
+        *frames = s->impl->frames;
+    });
+}
+
+int es_scorer_segments(es_scorer *s, int encode, int *count)
+{
+    return guard([&] {
+        require(s && count, "null argument");
+        require(encode >= 0 && encode < (int)s->impl->enc.size(), "no such encode");
+        *count = (int)s->impl->enc[encode].seg.size();
+    });
+}
+
+int es_scorer_segment(es_scorer *s, int encode, int seg, es_segment *out, uint64_t *hist)
+{
+    return guard([&] {
+        require(s && out, "null argument");
+        ScorerBase &b = *s->impl;
+        require(encode >= 0 && encode < (int)b.enc.size(), "no such encode");
+        auto &segs = b.enc[encode].seg;
+        require(seg < (int)segs.size(), "no such segment");
+        Accum a;
+        if (seg >= 0) {
+            a = segs[seg];
+            if (dynamic_cast<BatchedScorer *>(&b)) b.segment_hist(encode, seg, a);
+        } else {
+            for (size_t k = 0; k < segs.size(); k++) {
+                Accum x = segs[k];
+                if (dynamic_cast<BatchedScorer *>(&b)) b.segment_hist(encode, (int)k, x);
+                a.merge(x);
+            }
+        }
+        a.fill(out);
+        if (hist) std::memcpy(hist, a.hist.data(), a.hist.size() * 8);
+    });
+}
+
+int es_scorer_cambi(es_scorer *s, int encode, long n, double *cambi, double *cambi_source)
+{
+    return guard([&] {
+        require(s != nullptr, "null scorer");
+        ScorerBase &b = *s->impl;
+        require(encode >= 0 && encode < (int)b.enc.size(), "no such encode");
+        const EncodeState &e = b.enc[encode];
+        require(n >= 0 && n <= (long)e.cambi.size(), "more frames than scored");
+        if (cambi) std::copy(e.cambi.begin(), e.cambi.begin() + n, cambi);
+        if (cambi_source) std::copy(e.cambi_src.begin(), e.cambi_src.begin() + n, cambi_source);
+    });
+}
+
+int es_scorer_thresholds(es_scorer *s, int encode, long n, double *q)
+{
+    return guard([&] {
+        require(s && q, "null argument");
+        ScorerBase &b = *s->impl;
+        require(encode >= 0 && encode < (int)b.enc.size(), "no such encode");
+        const EncodeState &e = b.enc[encode];
+        require(n >= 0 && n <= (long)e.quantile.size(), "more frames than scored");
+        std::copy(e.quantile.begin(), e.quantile.begin() + n, q);
+    });
+}
+
+}  // extern "C"

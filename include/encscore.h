@@ -1,0 +1,220 @@
+/*
+ * libencscore_hip: the C ABI of encode-score's per-frame measure path on AMD Instinct MI355X (gfx950).
+ *
+ * Every function returns 0 on success or a nonzero status (ES_EINVAL, ES_EHIP, ES_ENOMEM, ES_ESTATE) and leaves a
+ * message for es_last_error().  Plain pointers and sizes only; device pointers are memory of the device es_init made
+ * current (es_malloc, or any HIP allocation of the same process).  Streams and events are opaque handles.
+ *
+ * Each declaration names the reference interface it replaces (file:line in wakamex/encode-score v0.1.0).
+ */
+#ifndef ENCSCORE_H
+#define ENCSCORE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ES_OK 0
+#define ES_EINVAL 1
+#define ES_EHIP 2
+#define ES_ENOMEM 3
+#define ES_ESTATE 4
+
+typedef struct es_stream_s *es_stream;
+typedef struct es_event_s *es_event;
+typedef struct es_cambi_s es_cambi;
+typedef struct es_scorer_s es_scorer;
+
+/* ---- runtime (replaces cu_check and Kernels, src/gpu.hpp:7-23, src/gpu.cpp:4-19) ---- */
+
+/* The message of the last failed call on this thread ("" when none). Replaces cu_check's message (src/gpu.hpp:7-13). */
+const char *es_last_error(void);
+/* Makes `device` current and loads the kernels. Replaces Kernels::load (src/gpu.cpp:4-6) and the context set-up of
+ * src/main.cpp:359-362. */
+int es_init(int device);
+int es_device_count(int *count);
+int es_device_synchronize(void);
+/* Replaces cuStreamCreate(CU_STREAM_NON_BLOCKING) / cuStreamDestroy (src/cambi.cpp:37, :85, src/main.cpp:368). */
+int es_stream_create(es_stream *out);
+int es_stream_destroy(es_stream s);
+int es_stream_synchronize(es_stream s);
+/* Replaces cuEventCreate / cuEventRecord / cuStreamWaitEvent / cuEventSynchronize (src/cambi.cpp:38, :103-104,
+ * src/main.cpp:573, :581); events here also time (es_event_elapsed_ms). */
+int es_event_create(es_event *out);
+int es_event_destroy(es_event e);
+int es_event_record(es_event e, es_stream s);
+int es_stream_wait_event(es_stream s, es_event e);
+int es_event_synchronize(es_event e);
+int es_event_elapsed_ms(es_event start, es_event end, float *ms);
+/* Replaces cuMemAlloc / cuMemFree / cuMemAllocHost / cuMemcpy*Async / cuMemsetD8Async (src/cambi.cpp:59-86,
+ * src/main.cpp:513-514). A NULL stream means the null stream and a synchronous copy. */
+int es_malloc(void **ptr, size_t bytes);
+int es_free(void *ptr);
+int es_host_alloc(void **ptr, size_t bytes);
+int es_host_free(void *ptr);
+int es_memcpy_htod(void *dst, const void *src, size_t bytes, es_stream s);
+int es_memcpy_dtoh(void *dst, const void *src, size_t bytes, es_stream s);
+int es_memcpy_dtod(void *dst, const void *src, size_t bytes, es_stream s);
+int es_memset(void *dst, int value, size_t bytes, es_stream s);
+
+/* ---- CAMBI (replaces class Cambi, src/cambi.hpp:9-35, src/cambi.cpp:31-184) ---- */
+
+/* The parameters Cambi::Cambi derives from the picture size (src/cambi.cpp:39-58); window_base is 65 in the
+ * reference (src/cambi.cpp:39). */
+typedef struct {
+    int w, h, depth, window, pad, mask_index;
+    int vlt, band_base, band_size;
+    int tvi[4], weight[4];
+    int sw[5], sh[5];
+    long long n[5], k[5];
+} es_cambi_info;
+int es_cambi_params(int w, int h, int depth, int window_base, es_cambi_info *out);
+
+/* Per-scale pooling figures of the last finished picture (Cambi::finish / topk_exact, src/cambi.cpp:137-181). */
+typedef struct {
+    unsigned long long sum_fx;   /* exact sum of the positive c-values, units of 2^-24 */
+    unsigned long long count;    /* positive c-values */
+    unsigned long long k;        /* clip((int)(0.6 n), 1, n) */
+    unsigned int thr_bits;       /* the k-th largest c-value's bits when count > k, else 0 */
+    unsigned long long above;    /* c-values strictly above it */
+    unsigned long long above_fx; /* their exact sum */
+    double pooled;
+} es_cambi_scale;
+
+/* Cambi(const Kernels&, int w, int h, int depth) (src/cambi.hpp:12). */
+int es_cambi_create(int w, int h, int depth, es_cambi **out);
+/* As es_cambi_create with the window's base constant (65 in the reference) and full c-value planes kept for
+ * es_cambi_cvalues when keep_cvalues is nonzero. */
+int es_cambi_create_ex(int w, int h, int depth, int window_base, int keep_cvalues, es_cambi **out);
+int es_cambi_destroy(es_cambi *c);
+/* Cambi::launch(CUdeviceptr img, int pitch, CUstream) (src/cambi.hpp:15-18): img is a device plane of uint16
+ * samples with `pitch` in pixels; it is read in stream order on `stream` and the rest runs on the instance's own
+ * stream, so the caller may overwrite the plane with work queued on `stream` afterwards. */
+int es_cambi_launch(es_cambi *c, const void *img, int pitch, es_stream stream);
+/* As es_cambi_launch for planes of `bytes` (1 or 2) per sample, shifted left by `shift` on reading. */
+int es_cambi_launch_ex(es_cambi *c, const void *img, int pitch, int bytes, int shift, es_stream stream);
+/* double Cambi::finish() (src/cambi.hpp:19-20): blocks and returns the score, clamped to 1000. */
+int es_cambi_finish(es_cambi *c, double *score);
+int es_cambi_scales(es_cambi *c, es_cambi_scale out[5]);
+/* Diagnostics of the last finished picture: the packed plane of a scale (value | mask << 15, sw x sh) and, when the
+ * instance keeps them, its c-values (sw x sh floats), into host memory. */
+int es_cambi_plane(es_cambi *c, int scale, uint16_t *host_out);
+int es_cambi_cvalues(es_cambi *c, int scale, float *host_out);
+
+/* ---- window energies and lasting texture (replace the launches of window_stats and persist_stats,
+ *      src/main.cpp:539-542, :558-566, kernels src/kernels.cu:346-347, :435) ---- */
+
+/* window_stats(yr, yd, up, w, h, mid_r, mid_d, out): for win x win windows (16 in the reference) of the pair of
+ * uint16 planes (pitch w), 12 exact sums per window into `sums` (int64, windows = (w / win) * (h / win)):
+ * [F_r^2, M_r^2, F_d^2, M_d^2, F_r F_d, M_r M_d, GX_r^2, GY_r^2, GX_r GY_r, GX_d^2, GY_d^2, GX_d GY_d] with F = 9 fine,
+ * M = 81 mid, GX/GY = 9 times box3's central differences; mid_r / mid_d (either may be NULL) receive M as int32 on
+ * the area the windows cover.  fp64 != 0 runs the double-precision form of the same arithmetic. */
+int es_window_stats(const uint16_t *yr, const uint16_t *yd, int up, int w, int h, int win, int32_t *mid_r,
+                    int32_t *mid_d, int64_t *sums, int fp64, es_stream s);
+/* es_window_values: the reference's 8 figures per window (kernels.cu:340-341) from the 12 sums, on the host. */
+int es_window_values(const int64_t *sums, int windows, double *values);
+/* persist_stats(ring, plane, centre, mid_d, w, out): `ring` holds 13 int32 mid planes `plane` elements apart, the
+ * centre frame's at slot `centre` (offset o at slot (centre + o + 13) % 13); 5 exact sums per window into `sums`:
+ * [M_r(c) ST, M_d(c) ST, SA SB, SA^2, SB^2] with SA/SB the sums at even/odd offsets and ST = SA + SB. */
+int es_persist_stats(const int32_t *ring, size_t plane, int centre, const int32_t *mid_d, int w, int h, int win,
+                     int64_t *sums, es_stream s);
+
+/* ---- selects (replace the radix select of src/kernels.cu:71-84 + src/cambi.cpp:144-157, quantile of
+ *      src/main.cpp:279-289 and Histogram::quantile of src/stats.hpp:21-30) ---- */
+
+/* The key of 0-based ascending rank `rank` among n device keys, into host *out. */
+int es_select_rank_u32(const uint32_t *keys, size_t n, size_t rank, uint32_t *out, es_stream s);
+int es_select_rank_u64(const uint64_t *keys, size_t n, size_t rank, uint64_t *out, es_stream s);
+/* Histogram::quantile of nhist device histograms of 20001 uint64 bins each, for nq quantiles: host out[h * nq + j]
+ * (NaN for an empty histogram) and host counts[h] (may be NULL). */
+int es_hist_quantiles(const uint64_t *hists, int nhist, const double *qs, int nq, double *out, uint64_t *counts);
+
+/* ---- the whole measure path of one source against its encodes (src/main.cpp:516-643 queue/collect,
+ *      src/stats.hpp Accum) ---- */
+
+#define ES_PLAIN 0   /* one launch per picture and per kernel, FP64 energies, collect on the host */
+#define ES_BATCHED 1 /* many frames per launch, integer energies, collect and selects on the device, graph replay */
+
+typedef struct {
+    int w, h;       /* the encode's picture; the source is centre-cropped to it (offsets (W - w) / 2 & ~1) */
+    int length;     /* frames of the encode; it leaves the run after them */
+    int pitch;      /* samples per row of its planes */
+    int bytes;      /* 1 or 2 bytes per sample */
+    int shift;      /* left shift of its samples to the pipeline depth */
+} es_encode_desc;
+
+typedef struct {
+    int mode;         /* ES_PLAIN or ES_BATCHED */
+    int src_w, src_h, src_pitch, src_bytes, src_shift;
+    int depth;        /* pipeline depth: 10, or 8 when every file is 8-bit (src/main.cpp:382, src/cambi.cpp:93) */
+    int win;          /* window of the energies: 16 (reference), 32 or 64 */
+    int per_segment;  /* frames per segment (src/main.cpp:434) */
+    int window_base;  /* CAMBI window base constant, 65 (src/cambi.cpp:39) */
+    int batch;        /* frames per batch in the batched form */
+    int tile_h;       /* c-value tile height, 0 = default */
+    int use_graph;    /* batched form: capture the per-batch pipeline once and replay it */
+    int n_encodes;
+    const es_encode_desc *encodes;
+} es_scorer_config;
+
+/* One segment of one encode (Accum, src/stats.hpp:36-82, measure fields), with its histograms' quantiles and the
+ * score parts (fidelity, from VMAF, is outside this path: left out of the weighted mean as an unmeasured part is). */
+typedef struct {
+    uint64_t frames;
+    double cambi, cambi_source;
+    double all_r[3], all_d[3], st_r[3], st_d[3];
+    double sh_all_r[2], sh_all_x[2], sh_st_r[2], sh_st_x[2];
+    double own, mine;
+    uint64_t n_amount[3], n_matching[2], n_lasting;
+    double median_amount[3], tenth_amount[3], median_matching[2], tenth_matching[2], median_lasting, tenth_lasting;
+    double texture, banding, grain, score;
+} es_segment;
+
+int es_scorer_create(const es_scorer_config *cfg, es_scorer **out);
+int es_scorer_destroy(es_scorer *s);
+/* Queues `nframes` frame steps: src[i] is the source's device plane of step i (the full picture), enc[i * n_encodes
+ * + e] encode e's (ignored once the encode has ended). Device planes must stay unchanged until es_scorer_sync. */
+int es_scorer_submit(es_scorer *s, int nframes, const void *const *src, const void *const *enc, es_stream stream);
+/* Waits for the queued work and folds its figures into the segments. */
+int es_scorer_sync(es_scorer *s);
+int es_scorer_frames(es_scorer *s, long *frames);
+int es_scorer_segments(es_scorer *s, int encode, int *count);
+/* Segment `seg` of encode `encode`, or the whole run merged when seg < 0; hist (may be NULL) receives its 6
+ * histograms (amount x3, matching x2, lasting), 20001 bins each. */
+int es_scorer_segment(es_scorer *s, int encode, int seg, es_segment *out, uint64_t *hist);
+/* Per-frame CAMBI of the encode and of the source cropped to it, for frames [0, n). */
+int es_scorer_cambi(es_scorer *s, int encode, long n, double *cambi, double *cambi_source);
+/* Per-frame diagnostics of the last synced frames: the 12 window sums of (encode, frame) when still held, the
+ * structure threshold of each frame (NaN when not kept). */
+int es_scorer_thresholds(es_scorer *s, int encode, long n, double *q);
+
+/* ---- synthetic inputs (the generator of the benchmark configurations; no counterpart in the reference, which
+ *      decodes files) ---- */
+
+typedef struct {
+    int kind;        /* 0: gradient + radial ramp with grain (config 1); 1: 64x64 tile mosaic (configs 2-5) */
+    uint64_t seed;   /* content and grain seed, shared by a source and its encodes */
+    int salt;        /* per-file seed of the block choice */
+    int full_w, full_h;   /* the source's picture, in whose coordinates content is defined */
+    int crop_x, crop_y, w, h;
+    int depth;       /* 10: codes in [64, 940]; 8: codes in [16, 235] */
+    /* distortions (0 = off) */
+    int grain_q8;        /* grain retention in 1/256 (256 = the source's grain) */
+    int blur_q8;         /* Gaussian blur sigma in 1/256 px */
+    int quant_step;      /* quantise to multiples of this many codes */
+    int quant_where;     /* 0 everywhere, 1 gradient tiles, 2 dark tiles */
+    int block;           /* block-mean size (8 or 16) */
+    int block_pct;       /* percentage of blocks averaged */
+    int box3;            /* config 1's encode: 3x3 box mean of the 4-code-quantised ramp, no grain */
+} es_synth_desc;
+/* Frame `frame` of the described file into a device plane (pitch w samples, `bytes` 1 or 2 per sample). */
+int es_synth_frame(const es_synth_desc *d, int frame, void *dev_out, int bytes, es_stream s);
+/* The same frame computed on the host (bit-identical), into host memory. */
+int es_synth_frame_host(const es_synth_desc *d, int frame, void *host_out, int bytes, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

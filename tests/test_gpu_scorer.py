@@ -1,0 +1,134 @@
+"""The whole measure path (plain per-frame form and batched form) against the oracle's run of src/main.cpp's
+queue/collect: per-frame CAMBI bit-exact, segment totals and histograms exact."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+W, H = 256, 160
+
+
+def frames_of(es, n, w=W, h=H, seed=77, enc_kw=None, crop=(0, 0), full=(W, H), src_depth=10):
+    sd = es.SynthDesc(kind=1, seed=seed, full_w=full[0], full_h=full[1], w=full[0], h=full[1], depth=src_depth,
+                      grain_q8=256)
+    kw = dict(grain_q8=0, quant_step=2, quant_where=2, block=8, block_pct=50)
+    kw.update(enc_kw or {})
+    ed = es.SynthDesc(kind=1, seed=seed, salt=1, full_w=full[0], full_h=full[1], crop_x=crop[0], crop_y=crop[1], w=w,
+                      h=h, depth=10, **kw)
+    src = [es.synth_frame_host(sd, f, bytes=1 if src_depth == 8 else 2) for f in range(n)]
+    enc = [es.synth_frame_host(ed, f) for f in range(n)]
+    return src, enc
+
+
+def oracle_run(orc, src, enc, per_segment, win=16, shift=0, crop=None):
+    h, w = enc[0].shape
+    run = orc.Run(w, h, win=win, per_segment=per_segment, threads=8)
+    cam, cam_src = [], []
+    for s, e in zip(src, enc):
+        s = s.astype(np.uint16) << shift
+        if crop is not None:
+            s = s[crop[1]:crop[1] + h, crop[0]:crop[0] + w]
+        a, b = run.frame(s, e)
+        cam.append(a)
+        cam_src.append(b)
+    return run, np.array(cam), np.array(cam_src)
+
+
+def compare_segment(got, ref_summary, exact=True):
+    ref, _ = ref_summary
+    for k, v in ref.items():
+        g = got[k]
+        if isinstance(v, list):
+            for a, b in zip(g, v):
+                assert (np.isnan(a) and np.isnan(b)) or a == b, k
+        else:
+            assert (isinstance(v, float) and np.isnan(v) and np.isnan(g)) or g == v, k
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched", "graph"])
+def test_scorer_matches_oracle(gpu, orc, mode):
+    n, per_segment = 30, 11
+    print(f"case scorer {mode}", flush=True)
+    src, enc = frames_of(gpu, n)
+    run, cam, cam_src = oracle_run(orc, src, enc, per_segment)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    sc = gpu.Scorer(gpu.ES_PLAIN if mode == "plain" else gpu.ES_BATCHED, W, H,
+                    [gpu.EncodeDesc(W, H, n, W, 2, 0)], per_segment=per_segment, batch=8, use_graph=mode == "graph")
+    sc.submit([d.ptr for d in ds[:13]], [[d.ptr] for d in de[:13]])
+    sc.submit([d.ptr for d in ds[13:]], [[d.ptr] for d in de[13:]])
+    sc.sync()
+    a, b = sc.cambi(0, n)
+    assert np.array_equal(a, cam) and np.array_equal(b, cam_src)
+    assert sc.segments(0) == run.segments()
+    for s in range(run.segments()):
+        got, hist = sc.segment(0, s, with_hist=True)
+        ref = orc.accum_summary(run.segment(s))
+        compare_segment(got, ref)
+        assert np.array_equal(hist, ref[1])
+    got = sc.segment(0, -1)
+    compare_segment(got, orc.accum_summary(run.merged()))
+
+
+def test_batched_uneven_lengths_and_crops(gpu, orc):
+    """Encodes of different crops and lengths in one run equal each (source crop, encode) pair scored alone."""
+    n = 26
+    fw, fh = 320, 192
+    specs = [((320, 192), 26, dict(blur_q8=154, grain_q8=256)), ((320, 128), 19, dict(quant_step=2, grain_q8=256)),
+             ((288, 192), 14, dict(grain_q8=0)), ((320, 128), 22, dict(block=16, block_pct=50, grain_q8=256))]
+    sd = gpu.SynthDesc(kind=1, seed=99, full_w=fw, full_h=fh, w=fw, h=fh, depth=10, grain_q8=256)
+    src = [gpu.synth_frame_host(sd, f) for f in range(n)]
+    encs, descs = [], []
+    for i, ((w, h), length, kw) in enumerate(specs):
+        cx, cy = ((fw - w) // 2) & ~1, ((fh - h) // 2) & ~1
+        ed = gpu.SynthDesc(kind=1, seed=99, salt=i, full_w=fw, full_h=fh, crop_x=cx, crop_y=cy, w=w, h=h, depth=10, **kw)
+        encs.append([gpu.synth_frame_host(ed, f) for f in range(length)])
+        descs.append(gpu.EncodeDesc(w, h, length, w, 2, 0))
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [[gpu.DeviceArray.from_host(f) for f in e] for e in encs]
+    for graph in (False, True):
+        sc = gpu.Scorer(gpu.ES_BATCHED, fw, fh, descs, per_segment=9, batch=6, use_graph=graph)
+        ptrs = [[de[e][f].ptr if f < len(de[e]) else ds[0].ptr for e in range(4)] for f in range(n)]
+        sc.submit([d.ptr for d in ds], ptrs)
+        sc.sync()
+        for e, ((w, h), length, _) in enumerate(specs):
+            cx, cy = ((fw - w) // 2) & ~1, ((fh - h) // 2) & ~1
+            run, cam, cam_src = oracle_run(orc, src[:length], encs[e], 9, crop=(cx, cy))
+            a, b = sc.cambi(e, length)
+            assert np.array_equal(a, cam) and np.array_equal(b, cam_src), (graph, e)
+            for s in range(run.segments()):
+                got, hist = sc.segment(e, s, with_hist=True)
+                ref = orc.accum_summary(run.segment(s))
+                compare_segment(got, ref)
+                assert np.array_equal(hist, ref[1])
+
+
+def test_plain_and_batched_agree_on_8bit_source(gpu, orc):
+    """An 8-bit source read as 1-byte samples shifted by 2 against a 10-bit encode (pipeline depth 10)."""
+    n = 16
+    src, enc = frames_of(gpu, n, src_depth=8, enc_kw=dict(grain_q8=128))
+    run, cam, cam_src = oracle_run(orc, src, enc, 1440, shift=2)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    for mode in (gpu.ES_PLAIN, gpu.ES_BATCHED):
+        sc = gpu.Scorer(mode, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], src_bytes=1, src_shift=2, batch=16)
+        sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
+        sc.sync()
+        a, b = sc.cambi(0, n)
+        assert np.array_equal(a, cam) and np.array_equal(b, cam_src)
+        compare_segment(sc.segment(0, 0), orc.accum_summary(run.segment(0)))
+
+
+def test_synth_device_equals_host(gpu):
+    descs = [gpu.SynthDesc(kind=0, seed=0x5EED0001, full_w=480, full_h=270, w=480, h=270, depth=10, grain_q8=256),
+             gpu.SynthDesc(kind=0, seed=0x5EED0001, full_w=480, full_h=270, w=480, h=270, depth=10, box3=1),
+             gpu.SynthDesc(kind=1, seed=5, full_w=400, full_h=300, crop_x=40, crop_y=20, w=320, h=240, depth=10,
+                           grain_q8=100, blur_q8=300, quant_step=4, quant_where=1, block=16, block_pct=30),
+             gpu.SynthDesc(kind=1, seed=6, full_w=256, full_h=144, w=256, h=144, depth=8, grain_q8=256)]
+    for d in descs:
+        for frame in (0, 7):
+            bytes_ = 1 if d.depth == 8 else 2
+            host = gpu.synth_frame_host(d, frame, bytes=bytes_)
+            dev = gpu.DeviceArray(host.shape, host.dtype)
+            gpu.synth_frame(d, frame, dev, bytes=bytes_)
+            assert np.array_equal(dev.host(), host)
