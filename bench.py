@@ -1,0 +1,254 @@
+"""Benchmark of the per-frame measure path on one MI355X.
+
+A step is one frame step: the source's frame and every active encode's frame through CAMBI (all scales, top-k), the
+window energies, the lasting texture and the collect.  Inputs are synthetic and generated into device memory before
+the timed region.  `value` is the batched form's frame steps per second (graph replay, integer energies); the plain
+per-frame-launch FP64 form is timed on the same frames in the same process.  The CPU baseline is the oracle run on
+the host cores over a bounded sample of the same frames.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1..5]
+"""
+import argparse
+import importlib
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "scored frames/s per encode, 1080p & 4K 10-bit; % HBM roofline; batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def config(n, es, frames):
+    """(source desc, [(encode desc, length)], source bytes, shift, workload text) of configuration n."""
+    mk = lambda **kw: es.SynthDesc(**kw)
+    seed = 0x5EED0000 + n
+    if n == 1:
+        w, h = 1920, 1080
+        src = mk(kind=0, seed=seed, full_w=w, full_h=h, w=w, h=h, depth=10, grain_q8=256)
+        enc = [(mk(kind=0, seed=seed, salt=1, full_w=w, full_h=h, w=w, h=h, depth=10, box3=1), frames)]
+        return src, enc, 2, 0, "C1 plain: 1 source + 1 encode, 1920x1080 10-bit ramp + grain vs 4-code banded box3"
+    if n == 2:
+        w, h = 3840, 2160
+        src = mk(kind=1, seed=seed, full_w=w, full_h=h, w=w, h=h, depth=10, grain_q8=256)
+        enc = [(mk(kind=1, seed=seed, salt=1, full_w=w, full_h=h, w=w, h=h, depth=10, grain_q8=0, quant_step=2,
+                   quant_where=2, block=8, block_pct=50), frames)]
+        return src, enc, 2, 0, "C2: 1 source + 1 encode, 3840x2160 10-bit tile mosaic, encode deblocked/degrained/dark-quantised"
+    if n == 3:
+        w, h = 1920, 1080
+        src = mk(kind=1, seed=seed, full_w=w, full_h=h, w=w, h=h, depth=10, grain_q8=256)
+        crops = [(1920, 1080), (1920, 800), (1920, 1040), (1800, 1080)]
+        dist = [dict(blur_q8=154, grain_q8=256), dict(quant_step=2, grain_q8=256), dict(grain_q8=0),
+                dict(block=16, block_pct=50, grain_q8=256)]
+        lens = [1.0, 1100 / 1440, 720 / 1440, 333 / 1440]
+        enc = []
+        for i, ((cw, ch), d, l) in enumerate(zip(crops, dist, lens)):
+            cx, cy = ((w - cw) // 2) & ~1, ((h - ch) // 2) & ~1
+            enc.append((mk(kind=1, seed=seed, salt=i, full_w=w, full_h=h, crop_x=cx, crop_y=cy, w=cw, h=ch, depth=10, **d),
+                        max(13, int(frames * l))))
+        return src, enc, 2, 0, "C3: 1920x1080 source + 4 encodes of uneven crops and lengths"
+    if n == 4:
+        w, h = 1920, 1080
+        src = mk(kind=1, seed=seed, full_w=w, full_h=h, w=w, h=h, depth=8, grain_q8=256)
+        dist = [dict(grain_q8=128), dict(quant_step=4, quant_where=1, grain_q8=256), dict(block=8, block_pct=25, grain_q8=256)]
+        enc = [(mk(kind=1, seed=seed, salt=i, full_w=w, full_h=h, w=w, h=h, depth=10, **d), frames) for i, d in enumerate(dist)]
+        return src, enc, 1, 2, "C4: 8-bit 1920x1080 source (1-byte samples, shift 2) + 3 10-bit encodes, graph replay"
+    if n == 5:
+        w, h = 3840, 1600
+        src = mk(kind=1, seed=seed, full_w=w, full_h=h, w=w, h=h, depth=10, grain_q8=256)
+        rng = np.random.default_rng(seed)
+        enc = []
+        for i in range(8):
+            d = dict(blur_q8=int(256 * rng.uniform(0.3, 1.2)), quant_step=int(rng.choice([1, 2, 4])),
+                     grain_q8=int(256 * rng.uniform(0, 1)))
+            length = max(13, int(frames * (1440 - i * (940 / 7)) / 1440))
+            enc.append((mk(kind=1, seed=seed, salt=i, full_w=w, full_h=h, w=w, h=h, depth=10, **d), length))
+        return src, enc, 2, 0, "C5: 3840x1600 source + 8 encodes, uneven lengths"
+    raise SystemExit(f"no configuration {n}")
+
+
+def load_oracle():
+    spec = importlib.util.spec_from_file_location("es_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=240)
+    ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--tile-h", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=48)
+    ap.add_argument("--cpu-frames", type=int, default=-1, help="frames of the CPU baseline sample (-1: per config)")
+    ap.add_argument("--pool", type=int, default=512, help="distinct frames generated per file at most")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py runs on one GPU only: this path has no multi-GPU form here (asked for {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    es = importlib.import_module("encode-score_amd")
+    es.init(0)
+    total = args.warmup + args.steps
+    src_d, encs, sbytes, sshift, workload = config(args.config, es, total)
+    W, H = src_d.w, src_d.h
+    pool = min(total, args.pool)
+    print(f"[bench] config {args.config}: {workload}; generating {pool} frames per file", file=sys.stderr, flush=True)
+
+    def gen(desc, nbytes):
+        frames = []
+        for f in range(pool):
+            d = es.DeviceArray((desc.h, desc.w), np.uint8 if nbytes == 1 else np.uint16)
+            es.synth_frame(desc, f, d, bytes=nbytes)
+            frames.append(d)
+        return frames
+
+    src_frames = gen(src_d, sbytes)
+    enc_frames = [gen(d, 2) for d, _ in encs]
+    es.synchronize()
+    descs = [es.EncodeDesc(d.w, d.h, length, d.w, 2, 0) for d, length in encs]
+
+    def ptrs(f0, n):
+        s = [src_frames[f % pool].ptr for f in range(f0, f0 + n)]
+        e = [[enc_frames[i][f % pool].ptr for i in range(len(encs))] for f in range(f0, f0 + n)]
+        return s, e
+
+    def timed(scorer, f0, n):
+        es.synchronize()
+        start, stop = es.Event(), es.Event()
+        stream = es.Stream()
+        t0 = time.perf_counter()
+        start.record(stream)
+        s, e = ptrs(f0, n)
+        scorer.submit(s, e, stream)
+        scorer.sync()
+        stop.record(stream)
+        stream.synchronize()
+        es.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, start.elapsed_ms(stop) / 1e3
+
+    # batched form (graph replay), with live per-stage events
+    bat = es.Scorer(es.ES_BATCHED, W, H, descs, batch=args.batch, tile_h=args.tile_h, use_graph=True, profile=True,
+                    src_bytes=sbytes, src_shift=sshift)
+    s, e = ptrs(0, args.warmup)
+    bat.submit(s, e)
+    bat.sync()
+    warm_stage, warm_rounds = bat.stage_ms()
+    print(f"[bench] warmup done ({args.warmup} frames)", file=sys.stderr, flush=True)
+    wall, dev = timed(bat, args.warmup, args.steps)
+    stage, rounds = bat.stage_ms()
+    stage = {k: stage[k] - warm_stage[k] for k in stage}
+    rounds -= warm_rounds
+    steps_per_s = args.steps / wall
+    encode_frames = sum(min(length, total) - min(length, args.warmup) for _, length in encs)
+    print(f"[bench] batched: {steps_per_s:.1f} steps/s ({wall * 1e3 / args.steps:.3f} ms/step)", file=sys.stderr, flush=True)
+
+    # the dominant kernel's roofline: algorithmic bytes per launch over its mean launch time (live HIP events)
+    n_src = W * H
+    crops = {(d.w, d.h) for d, _ in encs}
+    per_round_frames = args.batch
+    pix = {"src": sum(cw * ch for cw, ch in crops), "enc": sum(d.w * d.h for d, _ in encs)}
+    scale_px = lambda n: n * (1 + 1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
+    alg_bytes = {  # per launch (one round of `batch` frame steps)
+        "cambi_cvalues": 2 * scale_px(pix["src"] + pix["enc"]) * per_round_frames,
+        "cambi_base": (sbytes * pix["src"] + 2 * pix["enc"] + 2 * (pix["src"] + pix["enc"])) * per_round_frames,
+        "window_stats": (sbytes * pix["src"] + 2 * pix["enc"] + 4 * (len(crops) * 0 + pix["src"] + pix["enc"])) * per_round_frames,
+        "persist": (56 * pix["src"] + 8 * pix["enc"]) * per_round_frames,
+    }
+    launches = max(rounds, 1)
+    dominant = max(alg_bytes, key=lambda k: stage[k])
+    mean_ms = stage[dominant] / launches
+    achieved = alg_bytes[dominant] / (mean_ms / 1e3) / 1e9
+    # the whole step's input bytes (each luma plane read once) against HBM
+    b_in = sbytes * n_src + sum(2 * d.w * d.h for d, _ in encs)
+    step_gbs = b_in * steps_per_s / 1e9
+
+    result = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 3),
+        "unit": "frame steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic",
+        "config": {"workload": workload, "config": args.config, "width": W, "height": H, "encodes": len(encs),
+                   "batch": args.batch, "tile_h": args.tile_h or 32, "form": "batched, integer energies, graph replay"},
+        "encode_frames_per_s": round(encode_frames / wall, 3),
+        "device_ms_per_step": round(dev * 1e3 / args.steps, 4),
+        "hbm_frac_b_in": round(step_gbs / HBM_PEAK_GBS, 6),
+        "stage_ms_per_step": {k: round(v / args.steps, 5) for k, v in stage.items()},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "mean_launch_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes[dominant])},
+    }
+
+    # plain per-frame-launch FP64 form on the same frames
+    if not args.no_plain:
+        n_plain = min(args.plain_steps, args.steps)
+        pl = es.Scorer(es.ES_PLAIN, W, H, [es.EncodeDesc(d.w, d.h, min(d.length, args.warmup + n_plain), d.pitch, 2, 0)
+                                          for d in descs], src_bytes=sbytes, src_shift=sshift)
+        s, e = ptrs(0, args.warmup)
+        pl.submit(s, e)
+        pl.sync()
+        pwall, _ = timed(pl, args.warmup, n_plain)
+        plain_sps = n_plain / pwall
+        result["plain_fp64"] = {"value": round(plain_sps, 3), "unit": "frame steps/s", "steps": n_plain,
+                                "ms_per_step": round(pwall * 1e3 / n_plain, 4)}
+        result["batched_over_plain"] = round(steps_per_s / plain_sps, 3)
+        # the two forms must agree exactly on the frames both scored
+        agree = True
+        for i in range(len(encs)):
+            n = min(descs[i].length, args.warmup + n_plain)
+            a, _ = bat.cambi(i, n)
+            b, _ = pl.cambi(i, n)
+            agree &= bool(np.array_equal(a, b))
+        result["plain_equals_batched_cambi"] = agree
+        print(f"[bench] plain: {plain_sps:.2f} steps/s", file=sys.stderr, flush=True)
+
+    # CPU baseline: the oracle on the host cores, bounded sample of the same frames
+    if not args.no_cpu:
+        orc = load_oracle()
+        cpu_frames = args.cpu_frames if args.cpu_frames >= 0 else (16 if W * H > 4e6 else 24)
+        cpu_frames = min(cpu_frames, pool)
+        threads = min(16, os.cpu_count() or 1)
+        d0, _ = encs[0]
+        cx, cy = ((W - d0.w) // 2) & ~1, ((H - d0.h) // 2) & ~1
+        run = orc.Run(d0.w, d0.h, threads=threads)
+        frames = [(src_frames[f].host().astype(np.uint16) << sshift, enc_frames[0][f].host()) for f in range(cpu_frames)]
+        t0 = time.perf_counter()
+        cam = []
+        for s_, e_ in frames:
+            cam.append(run.frame(s_[cy:cy + d0.h, cx:cx + d0.w], e_)[0])
+        cpu_t = time.perf_counter() - t0
+        # pairs per second of the first encode, scaled to frame steps of this configuration's encodes
+        pair_s = cpu_frames / cpu_t
+        result["cpu_baseline"] = {"value": round(pair_s / len(encs), 4), "unit": "frame steps/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"oracle run of the first {cpu_frames} frames of encode 0 ({d0.w}x{d0.h}), "
+                                            f"{cpu_t:.1f} s; per-step rate divides by the {len(encs)} encode(s)"}
+        gpu_cam, _ = bat.cambi(0, min(cpu_frames, descs[0].length))
+        result["cambi_bit_exact_vs_oracle"] = bool(np.array_equal(gpu_cam, np.array(cam[: len(gpu_cam)])))
+        print(f"[bench] cpu: {pair_s:.3f} pair-frames/s on {threads} threads", file=sys.stderr, flush=True)
+
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -15,6 +15,7 @@ from .escore import (  # noqa: F401
     Scorer,
     SynthDesc,
     cambi_params,
+    copy_kernel,
     device_count,
     hist_quantiles,
     init,

@@ -15,6 +15,8 @@
 #include "device_util.hpp"
 #include "energy.hpp"
 
+#include <algorithm>
+
 namespace es {
 
 // Window figures from the exact sums (src/kernels.cu:421-425), as the oracle computes them.
@@ -323,6 +325,21 @@ void launch_hist_quantiles(const unsigned long long *hists, int nhist, const dou
     if (nhist <= 0) return;
     hipLaunchKernelGGL(hist_quantile_kernel, dim3(nhist), dim3(256), 0, st, hists, qs, nq, out, counts);
     hip_check(hipGetLastError(), "histogram quantiles");
+}
+
+// A plain copy of 16-byte vectors, grid-strided, for calibrating HBM byte counters against a known size.
+__global__ __launch_bounds__(256) void copy_kernel(uint4 *dst, const uint4 *src, size_t n)
+{
+    for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    const size_t n = bytes / 16;
+    if (!n) return;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, st, (uint4 *)dst, (const uint4 *)src, n);
+    hip_check(hipGetLastError(), "copy");
 }
 
 template <typename K>

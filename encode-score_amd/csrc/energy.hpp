@@ -54,6 +54,7 @@ void launch_hist_quantiles(const unsigned long long *hists, int nhist, const dou
                            unsigned long long *counts, hipStream_t st);
 void launch_select_u64(const unsigned long long *keys, long long n, long long rank, unsigned long long *out,
                        hipStream_t st);
+void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 void launch_select_u32(const unsigned int *keys, long long n, long long rank, unsigned int *out, hipStream_t st);
 
 }  // namespace es

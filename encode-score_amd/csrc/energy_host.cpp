@@ -87,6 +87,15 @@ int es_persist_stats(const int32_t *ring, size_t plane, int centre, const int32_
     });
 }
 
+int es_copy_kernel(void *dst, const void *src, size_t bytes, es_stream s)
+{
+    return guard([&] {
+        ensure_init();
+        require(dst && src && bytes % 16 == 0, "bad argument");
+        launch_copy(dst, src, bytes, hs(s));
+    });
+}
+
 int es_select_rank_u32(const uint32_t *keys, size_t n, size_t rank, uint32_t *out, es_stream s)
 {
     return guard([&] {

@@ -281,6 +281,9 @@ public:
         require(B <= 64, "batch must be at most 64 frames");
         R = B + 16;
         tile_h = c.tile_h > 0 ? c.tile_h : 32;
+        int cus = 0;
+        hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "device attributes");
+        cv_blocks = std::max(1, cus) * 8;
         const int C = (int)crops.size(), E = (int)enc.size();
         roles = C + E;
         P = B * roles;
@@ -378,13 +381,17 @@ public:
         res_bytes = align((size_t)P * 8) + align((size_t)B * E * 12 * 8) + align((size_t)B * E * 4 * 8) +
                     align((size_t)B * E * 3 * 8) + align((size_t)B * E * 8);
         for (int k = 0; k < 2; k++) {
-            stage[k].alloc(desc_bytes);
+            stage_buf[k].alloc(desc_bytes);
             res[k].alloc(res_bytes);
             hip_check(hipEventCreateWithFlags(&done[k], hipEventDisableTiming), "creating an event");
         }
         hip_check(hipStreamCreateWithFlags(&work, hipStreamNonBlocking), "creating a stream");
         hip_check(hipEventCreateWithFlags(&joined, hipEventDisableTiming), "creating an event");
         use_graph = c.use_graph != 0;
+        profile = c.profile != 0;
+        if (profile)
+            for (auto &row : ev)
+                for (auto &e : row) hip_check(hipEventCreate(&e), "creating an event");
     }
 
     ~BatchedScorer() override
@@ -396,13 +403,23 @@ public:
             }
             if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
+            for (int j = 0; j < ES_STAGES; j++) {
+                if (sexec[k][j]) (void)hipGraphExecDestroy(sexec[k][j]);
+                if (sgraph[k][j]) (void)hipGraphDestroy(sgraph[k][j]);
+            }
         }
         if (work) {
             (void)hipStreamSynchronize(work);
             (void)hipStreamDestroy(work);
         }
         if (joined) (void)hipEventDestroy(joined);
+        for (auto &row : ev)
+            for (auto &e : row)
+                if (e) (void)hipEventDestroy(e);
     }
+
+    double stage_ms[ES_STAGES] = {};
+    long profiled_rounds = 0;
 
     // The rounds run on the scorer's own stream (a graph cannot be captured on the null stream), ordered after the
     // caller's earlier work on `st`, and the caller's later work on `st` waits for them.
@@ -449,7 +466,7 @@ private:
         int n = 0;
     };
 
-    int B, R, P, roles, tile_h, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
+    int B, R, P, roles, tile_h, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
     std::vector<long long> max_px;
     std::vector<size_t> slot_off;
     DevBuf consts, planes, list, sums, counts, thist, thsum, tticket, topk, scores;
@@ -457,11 +474,13 @@ private:
     std::vector<EncDev> ed;
     DevBuf tot_int, tot_or, tot_last, quant, scratch, desc;
     size_t desc_bytes, res_bytes;
-    HostBuf stage[2], res[2];
+    HostBuf stage_buf[2], res[2];
     hipEvent_t done[2] = {}, joined = nullptr;
+    hipEvent_t ev[2][ES_STAGES + 1] = {};
+    bool profile = false;
     hipStream_t work = nullptr;
-    hipGraph_t graph[2] = {};
-    hipGraphExec_t exec[2] = {};
+    hipGraph_t graph[2] = {}, sgraph[2][ES_STAGES] = {};
+    hipGraphExec_t exec[2] = {}, sexec[2][ES_STAGES] = {};
     Pending pend[2];
     long rounds = 0;
     bool use_graph = false;
@@ -484,7 +503,7 @@ private:
 
     void fill(int p, long f0, int n, const void *const *src, const void *const *encp)
     {
-        char *h = stage[p].as<char>();
+        char *h = stage_buf[p].as<char>();
         std::memset(h, 0, desc_bytes);
         const int C = (int)crops.size(), E = (int)enc.size();
         CambiPic *pics = (CambiPic *)(h + o_pics());
@@ -585,30 +604,82 @@ private:
     }
 
     // The round's pipeline on `st`: the descriptor copy, the kernels and the copy of the results.
-    void pipeline(int p, hipStream_t st)
+    // Stage k of the round's pipeline on `st` (ES_STAGE_NAMES): the descriptor copy and clear, the CAMBI kernels,
+    // the energies, the lasting texture, the collect and the copy of the results.
+    void stage(int k, int p, hipStream_t st)
     {
         const int C = (int)crops.size(), E = (int)enc.size();
-        hip_check(hipMemcpyAsync(desc.p, stage[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
         const CambiPic *pics = dptr<CambiPic>(o_pics());
-        const int *first = dptr<int>(o_first());
-        unsigned *counter = counts.as<unsigned>() + (size_t)P * kScales;
-        launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
-        launch_cambi_base(pics, nullptr, P, max_base_tiles, st);
-        for (int s = 1; s < kScales; s++) launch_cambi_scale(pics, nullptr, P, max_px[s], s, st);
-        launch_cambi_cvalues(pics, first, P * kScales, 0, tile_h, 1, counter, 2048, st);
-        for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, 16, st);
-        launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st);
-        launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st);
-        launch_persist(dptr<PersistJob>(o_psrc()), B * C, max_w, max_h, cfg.win, 1, st);
-        launch_persist(dptr<PersistJob>(o_penc()), B * E, max_w, max_h, cfg.win, 2, st);
-        launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 0, st);
-        launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 1, st);
-        char *r = res[p].as<char>();
-        hip_check(hipMemcpyAsync(r + r_scores(), scores.p, (size_t)P * 8, hipMemcpyDeviceToHost, st), "results");
-        hip_check(hipMemcpyAsync(r + r_ti(), tot_int.p, tot_int.bytes, hipMemcpyDeviceToHost, st), "results");
-        hip_check(hipMemcpyAsync(r + r_to(), tot_or.p, tot_or.bytes, hipMemcpyDeviceToHost, st), "results");
-        hip_check(hipMemcpyAsync(r + r_tl(), tot_last.p, tot_last.bytes, hipMemcpyDeviceToHost, st), "results");
-        hip_check(hipMemcpyAsync(r + r_q(), quant.p, quant.bytes, hipMemcpyDeviceToHost, st), "results");
+        switch (k) {
+        case 0:
+            hip_check(hipMemcpyAsync(desc.p, stage_buf[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
+            launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
+            break;
+        case 1: launch_cambi_base(pics, nullptr, P, max_base_tiles, st); break;
+        case 2:
+            for (int s = 1; s < kScales; s++) launch_cambi_scale(pics, nullptr, P, max_px[s], s, st);
+            break;
+        case 3:
+            launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, 0, tile_h, 1, counts.as<unsigned>() + (size_t)P * kScales,
+                                 cv_blocks, st);
+            break;
+        case 4:
+            for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, 16, st);
+            break;
+        case 5: launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st); break;
+        case 6: launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st); break;
+        case 7:
+            launch_persist(dptr<PersistJob>(o_psrc()), B * C, max_w, max_h, cfg.win, 1, st);
+            launch_persist(dptr<PersistJob>(o_penc()), B * E, max_w, max_h, cfg.win, 2, st);
+            break;
+        case 8:
+            launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 0, st);
+            launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 1, st);
+            break;
+        case 9: {
+            char *r = res[p].as<char>();
+            hip_check(hipMemcpyAsync(r + r_scores(), scores.p, (size_t)P * 8, hipMemcpyDeviceToHost, st), "results");
+            hip_check(hipMemcpyAsync(r + r_ti(), tot_int.p, tot_int.bytes, hipMemcpyDeviceToHost, st), "results");
+            hip_check(hipMemcpyAsync(r + r_to(), tot_or.p, tot_or.bytes, hipMemcpyDeviceToHost, st), "results");
+            hip_check(hipMemcpyAsync(r + r_tl(), tot_last.p, tot_last.bytes, hipMemcpyDeviceToHost, st), "results");
+            hip_check(hipMemcpyAsync(r + r_q(), quant.p, quant.bytes, hipMemcpyDeviceToHost, st), "results");
+            break;
+        }
+        }
+    }
+
+    hipGraphExec_t capture(int k0, int k1, int p, hipStream_t st, hipGraph_t *g)
+    {
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+        for (int k = k0; k < k1; k++) stage(k, p, st);
+        hip_check(hipStreamEndCapture(st, g), "graph capture");
+        hipGraphExec_t e;
+        hip_check(hipGraphInstantiate(&e, *g, nullptr, nullptr, 0), "graph instantiate");
+        return e;
+    }
+
+    // The round on `st`: eager launches, one graph, or (profiled) one graph per stage with events between them.
+    void pipeline(int p, hipStream_t st)
+    {
+        if (!use_graph) {
+            for (int k = 0; k < ES_STAGES; k++) {
+                if (profile) hip_check(hipEventRecord(ev[p][k], st), "stage event");
+                stage(k, p, st);
+            }
+            if (profile) hip_check(hipEventRecord(ev[p][ES_STAGES], st), "stage event");
+            return;
+        }
+        if (!profile) {
+            if (!exec[p]) exec[p] = capture(0, ES_STAGES, p, st, &graph[p]);
+            hip_check(hipGraphLaunch(exec[p], st), "graph launch");
+            return;
+        }
+        for (int k = 0; k < ES_STAGES; k++) {
+            if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, st, &sgraph[p][k]);
+            hip_check(hipEventRecord(ev[p][k], st), "stage event");
+            hip_check(hipGraphLaunch(sexec[p][k], st), "graph launch");
+        }
+        hip_check(hipEventRecord(ev[p][ES_STAGES], st), "stage event");
     }
 
     void round(int n, const void *const *src, const void *const *encp, hipStream_t st)
@@ -616,17 +687,7 @@ private:
         const int p = (int)(rounds % 2);
         if (pend[p].n) fold(p);
         fill(p, frames, n, src, encp);
-        if (use_graph) {
-            if (!exec[p]) {
-                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
-                pipeline(p, st);
-                hip_check(hipStreamEndCapture(st, &graph[p]), "graph capture");
-                hip_check(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "graph instantiate");
-            }
-            hip_check(hipGraphLaunch(exec[p], st), "graph launch");
-        } else {
-            pipeline(p, st);
-        }
+        pipeline(p, st);
         hip_check(hipEventRecord(done[p], st), "event");
         pend[p].f0 = frames;
         pend[p].n = n;
@@ -637,6 +698,14 @@ private:
     void fold(int p)
     {
         hip_check(hipEventSynchronize(done[p]), "waiting for a round");
+        if (profile) {
+            for (int k = 0; k < ES_STAGES; k++) {
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, ev[p][k], ev[p][k + 1]), "stage time");
+                stage_ms[k] += ms;
+            }
+            profiled_rounds++;
+        }
         const char *r = res[p].as<char>();
         const double *sc = (const double *)(r + r_scores());
         const long long *ti = (const long long *)(r + r_ti());
@@ -758,6 +827,17 @@ int es_scorer_cambi(es_scorer *s, int encode, long n, double *cambi, double *cam
         require(n >= 0 && n <= (long)e.cambi.size(), "more frames than scored");
         if (cambi) std::copy(e.cambi.begin(), e.cambi.begin() + n, cambi);
         if (cambi_source) std::copy(e.cambi_src.begin(), e.cambi_src.begin() + n, cambi_source);
+    });
+}
+
+int es_scorer_stage_ms(es_scorer *s, double *ms, long *rounds)
+{
+    return guard([&] {
+        require(s && ms, "null argument");
+        auto *b = dynamic_cast<BatchedScorer *>(s->impl.get());
+        require(b != nullptr, "stage times are kept by the batched form");
+        for (int k = 0; k < ES_STAGES; k++) ms[k] = b->stage_ms[k];
+        if (rounds) *rounds = b->profiled_rounds;
     });
 }
 

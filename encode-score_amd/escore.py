@@ -46,7 +46,7 @@ class EncodeDesc(C.Structure):
 class ScorerConfig(C.Structure):
     _fields_ = [("mode", _i), ("src_w", _i), ("src_h", _i), ("src_pitch", _i), ("src_bytes", _i), ("src_shift", _i),
                 ("depth", _i), ("win", _i), ("per_segment", _i), ("window_base", _i), ("batch", _i), ("tile_h", _i),
-                ("use_graph", _i), ("n_encodes", _i), ("encodes", C.POINTER(EncodeDesc))]
+                ("use_graph", _i), ("n_encodes", _i), ("encodes", C.POINTER(EncodeDesc)), ("profile", _i)]
 
 
 class Segment(C.Structure):
@@ -108,6 +108,7 @@ for _n, _a in [
     ("es_scorer_frames", [_vp, C.POINTER(C.c_long)]), ("es_scorer_segments", [_vp, _i, C.POINTER(_i)]),
     ("es_scorer_segment", [_vp, _i, _i, C.POINTER(Segment), _vp]),
     ("es_scorer_cambi", [_vp, _i, C.c_long, _vp, _vp]), ("es_scorer_thresholds", [_vp, _i, C.c_long, _vp]),
+    ("es_scorer_stage_ms", [_vp, _vp, C.POINTER(C.c_long)]), ("es_copy_kernel", [_vp, _vp, _sz, _vp]),
     ("es_synth_frame", [C.POINTER(SynthDesc), _i, _vp, _i, _vp]),
     ("es_synth_frame_host", [C.POINTER(SynthDesc), _i, _vp, _i, _i]),
 ]:
@@ -317,10 +318,10 @@ class Scorer:
     """The measure path of one source against its encodes (src/main.cpp's queue/collect)."""
 
     def __init__(self, mode, src_w, src_h, encodes, depth=10, win=16, per_segment=1440, window_base=65, batch=16,
-                 tile_h=0, use_graph=False, src_pitch=None, src_bytes=2, src_shift=0):
+                 tile_h=0, use_graph=False, src_pitch=None, src_bytes=2, src_shift=0, profile=False):
         self._encodes = (EncodeDesc * len(encodes))(*encodes)
         cfg = ScorerConfig(mode, src_w, src_h, src_pitch or src_w, src_bytes, src_shift, depth, win, per_segment,
-                           window_base, batch, tile_h, int(use_graph), len(encodes), self._encodes)
+                           window_base, batch, tile_h, int(use_graph), len(encodes), self._encodes, int(profile))
         p = _vp()
         check(lib.es_scorer_create(C.byref(cfg), C.byref(p)))
         self.ptr = p.value
@@ -358,6 +359,15 @@ class Scorer:
         check(lib.es_scorer_cambi(self.ptr, e, n, a.ctypes.data, b.ctypes.data))
         return a, b
 
+    STAGES = ("upload", "cambi_base", "cambi_scale", "cambi_cvalues", "cambi_topk", "cambi_finish", "window_stats",
+              "persist", "collect", "results")
+
+    def stage_ms(self):
+        ms = np.zeros(len(self.STAGES))
+        rounds = C.c_long()
+        check(lib.es_scorer_stage_ms(self.ptr, ms.ctypes.data, C.byref(rounds)))
+        return dict(zip(self.STAGES, ms)), rounds.value
+
     def thresholds(self, e, n):
         q = np.empty(n)
         check(lib.es_scorer_thresholds(self.ptr, e, n, q.ctypes.data))
@@ -368,6 +378,10 @@ class Scorer:
             lib.es_scorer_destroy(self.ptr)
         except Exception:
             pass
+
+
+def copy_kernel(dst, src, nbytes, stream=None):
+    check(lib.es_copy_kernel(_ptr(dst), _ptr(src), nbytes, _ptr(stream)))
 
 
 def synth_frame(desc, frame, dev, bytes=2, stream=None):

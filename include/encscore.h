@@ -157,6 +157,7 @@ typedef struct {
     int use_graph;    /* batched form: capture the per-batch pipeline once and replay it */
     int n_encodes;
     const es_encode_desc *encodes;
+    int profile;      /* batched form: time each stage of every round with events (es_scorer_stage_ms) */
 } es_scorer_config;
 
 /* One segment of one encode (Accum, src/stats.hpp:36-82, measure fields), with its histograms' quantiles and the
@@ -186,9 +187,18 @@ int es_scorer_segments(es_scorer *s, int encode, int *count);
 int es_scorer_segment(es_scorer *s, int encode, int seg, es_segment *out, uint64_t *hist);
 /* Per-frame CAMBI of the encode and of the source cropped to it, for frames [0, n). */
 int es_scorer_cambi(es_scorer *s, int encode, long n, double *cambi, double *cambi_source);
+/* Batched form with profile set: the device milliseconds of each stage summed over the synced rounds, for
+ * ES_STAGES stages in the order ES_STAGE_NAMES lists, and the number of rounds. */
+#define ES_STAGES 10
+#define ES_STAGE_NAMES "upload,cambi_base,cambi_scale,cambi_cvalues,cambi_topk,cambi_finish,window_stats,persist,collect,results"
+int es_scorer_stage_ms(es_scorer *s, double *ms, long *rounds);
 /* Per-frame diagnostics of the last synced frames: the 12 window sums of (encode, frame) when still held, the
  * structure threshold of each frame (NaN when not kept). */
 int es_scorer_thresholds(es_scorer *s, int encode, long n, double *q);
+
+/* A plain copy kernel of `bytes` (a multiple of 16) between device buffers, for calibrating HBM byte counters; no
+ * counterpart in the reference. */
+int es_copy_kernel(void *dst, const void *src, size_t bytes, es_stream s);
 
 /* ---- synthetic inputs (the generator of the benchmark configurations; no counterpart in the reference, which
  *      decodes files) ---- */

@@ -132,3 +132,31 @@ def test_synth_device_equals_host(gpu):
             dev = gpu.DeviceArray(host.shape, host.dtype)
             gpu.synth_frame(d, frame, dev, bytes=bytes_)
             assert np.array_equal(dev.host(), host)
+
+
+def test_profiled_graph_rounds_match_eager(gpu):
+    """Stage events recorded inside the captured graph time every round and leave the results unchanged; several
+    replays with eager work and synchronisations between them give the eager results."""
+    n = 40
+    src, enc = frames_of(gpu, n, seed=5)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    out = {}
+    for graph, profile in ((False, False), (True, True)):
+        sc = gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], batch=4, use_graph=graph,
+                        profile=profile)
+        for f0 in range(0, n, 10):
+            sc.submit([d.ptr for d in ds[f0:f0 + 10]], [[d.ptr] for d in de[f0:f0 + 10]])
+            gpu.synchronize()
+            scratch = gpu.DeviceArray.from_host(np.arange(1000, dtype=np.int64))  # eager work between replays
+            assert scratch.host()[999] == 999
+        sc.sync()
+        out[graph] = (sc.cambi(0, n), sc.segment(0, -1, with_hist=True))
+        if profile:
+            ms, rounds = sc.stage_ms()
+            assert rounds == (n // 10) * 3 and all(v >= 0 for v in ms.values()) and ms["cambi_cvalues"] > 0
+    (a0, b0), (s0, h0) = out[False]
+    (a1, b1), (s1, h1) = out[True]
+    assert np.array_equal(a0, a1) and np.array_equal(b0, b1) and np.array_equal(h0, h1)
+    for k, v in s0.items():
+        assert (v == s1[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(s1[k])), k
