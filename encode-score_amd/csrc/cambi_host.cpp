@@ -28,7 +28,10 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     hip_check(hipMemcpy(consts_.p, &k, sizeof k, hipMemcpyHostToDevice), "cambi constants");
     planes_.alloc(total * 2);
     if (keep_) cvals_.alloc(total * 4);
-    list_.alloc(total * 4);
+    int ntiles = 0;
+    const long long cap = cambi_list_layout(info_.sw, info_.sh, tile_h_, loff_, toff_, &ntiles);
+    list_.alloc((size_t)cap * 4);
+    segs_.alloc((size_t)ntiles * 4);
     sums_.alloc(kScales * 8);
     counts_.alloc(kScales * 4);
     hist_.alloc(kScales * 512 * 4);
@@ -39,7 +42,10 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     hip_check(hipMemset(ticket_.p, 0, ticket_.bytes), "clearing");
     TopkJob jobs[kScales] = {};
     for (int s = 0; s < kScales; s++) {
-        jobs[s].list = list_.as<float>() + info_.off[s];
+        jobs[s].list = list_.as<float>() + loff_[s];
+        jobs[s].seg_count = segs_.as<unsigned>() + toff_[s];
+        jobs[s].nseg = cvalue_tiles(info_.sw[s], info_.sh[s], tile_h_);
+        jobs[s].seg_cap = kCvTileW * tile_h_;
         jobs[s].count = counts_.as<unsigned>() + s;
         jobs[s].k = (unsigned long long)info_.k[s];
         jobs[s].hist = hist_.as<unsigned>() + s * 512;
@@ -90,14 +96,16 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
         p->sw[s] = info_.sw[s];
         p->sh[s] = info_.sh[s];
         p->off[s] = info_.off[s];
+        p->loff[s] = loff_[s];
+        p->toff[s] = toff_[s];
     }
     p->planes = planes_.as<uint16_t>();
     p->cvals = keep_ ? cvals_.as<float>() : nullptr;
     p->list = list_.as<float>();
+    p->seg_count = segs_.as<unsigned>();
     p->sums = sums_.as<unsigned long long>();
     p->counts = counts_.as<unsigned>();
     p->consts = consts_.as<CambiConsts>();
-    p->lut = device_lut(info_.lut_size);
     const CambiPic *dpic = pic_.as<CambiPic>();
     hip_check(hipMemcpyAsync(pic_.p, p, sizeof *p, hipMemcpyHostToDevice, caller), "cambi descriptor");
     launch_cambi_base(dpic, nullptr, 1, base_tiles_, caller);

@@ -164,21 +164,24 @@ void launch_cambi_scale(const CambiPic *pics, const long long * /*px_first*/, in
 // end in the last).
 //
 // A pass keeps, for each of the 64 + 2 pad columns of the tile and its margins, the counts of the pass's B numbers in
-// the vertical window around the current row (u8, 4 per word, in LDS); lane i owns columns i and 64 + i.  Per row, a
-// wave-wide prefix sum over the columns of each pair of counts (16 bits each) gives every lane the window count
-// P(x + span) - P(x) of its output column, with one ds_bpermute for the far end; the counts go to LDS for the lane's
-// own lookups of p0 and the neighbours' counts.  No lane reads another lane's LDS words, so no barrier is needed in
-// the row loop.
+// the vertical window around the current row (u8, 4 per word, in LDS); lane i owns columns 2i and 2i + 1 and updates
+// them as rows enter and leave the window.  Per row, every lane reads all its count words at once, then the wave
+// runs one prefix sum per pair of numbers (16 bits each) over the column pairs, the scans of all pairs interleaved
+// step by step; the prefixes P(c) of every column go to LDS and a served pixel at output column x reads the window
+// count of each number it needs as P(x + span) - P(x).  The codes of the next row are loaded while the current row is
+// processed, the setup's loads go out eight rows at a time, and the c-value arithmetic is predicated rather than
+// branched.  Positive c-values go to the tile's own segment of the list at a wave-uniform offset.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int kCvMaxB = 32;
 constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
+constexpr int kCvCols = 130;    // prefix entries per number pair: columns 0 .. 128, padded
 
 struct CvShared {
-    uint32_t cc[kCvMaxB / 4][128];         // column counts: bins 4q .. 4q + 3 of column c in word cc[q][c]
-    uint32_t hh[kCvMaxB / 2][64];          // window counts: bins 2j, 2j + 1 of output column x in word hh[j][x]
-    uint16_t rank[kMaxBand + 2 * kDiffs];  // number of value v at rank[v + 4], kNone if not relevant
+    uint32_t cc[kCvMaxB / 4][128];            // column counts: numbers 4q .. 4q + 3 of column c in cc[q][c]
+    uint32_t pp[kCvMaxB / 2][kCvCols];        // prefix over columns: numbers 2j, 2j + 1 in pp[j][c]
+    uint16_t rank[kMaxBand + 2 * kDiffs];     // number of value v at rank[v + 4], kNone if not relevant
 };
-static_assert(sizeof(uint32_t) * (kCvMaxB / 2) * 64 >= 2 * kCvFlags, "flags live in hh during the setup");
+static_assert(sizeof(uint32_t) * (kCvMaxB / 2) * kCvCols >= 2 * kCvFlags, "flags live in pp during the setup");
 
 int cvalue_tiles(int w, int h, int tile_h) { return ((w + kCvTileW - 1) / kCvTileW) * ((h + tile_h - 1) / tile_h); }
 
@@ -187,100 +190,148 @@ struct CvTile {
     int w, h, x0, y0, rows;
     int pad, band_base, band_size, vlt;
     int tvi[kDiffs], weight[kDiffs];
-    const float *lut;
-    float *plane;           // full c-value plane or nullptr
-    float *list;
-    unsigned int *count;
+    float *plane;  // full c-value plane or nullptr
+    float *seg;    // this tile's segment of the list
 };
 
-__device__ __forceinline__ unsigned cv_code(const CvTile &t, int x, int y)
+// The raw packed sample at (x, y), or 0 (mask bit clear: never active) outside the picture.
+__device__ __forceinline__ unsigned cv_raw(const CvTile &t, int x, int y)
 {
-    if (x < 0 || x >= t.w || y < 0 || y >= t.h) return kNone;
-    const unsigned v = t.img[(size_t)y * t.w + x];
+    return (x >= 0 && x < t.w && y >= 0 && y < t.h) ? (unsigned)t.img[(size_t)y * t.w + x] : 0u;
+}
+
+// The value less band_base of an active (masked, in-band) packed sample, else kNone.
+__device__ __forceinline__ unsigned cv_decode(const CvTile &t, unsigned v)
+{
     const unsigned u = (v & 0x7FFFu) - (unsigned)t.band_base;
     return (v >> 15) && u < (unsigned)t.band_size ? u : kNone;
 }
 
 template <int B>
-__device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned long long &sum_fx)
+__device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
 {
-    constexpr int Q = B / 4;
-    const int lane = threadIdx.x, pad = t.pad, span = 2 * pad + 1, cw = kCvTileW + 2 * pad;
-    const int xl = t.x0 - pad;  // picture column of margin column 0
+    constexpr int Q = B / 4, W2 = B / 2;
+    const int lane = threadIdx.x, pad = t.pad, span = 2 * pad + 1;
+    const int c0 = 2 * lane, xl = t.x0 - pad + c0;  // this lane's margin columns c0, c0 + 1 at picture column xl
+    const bool col0 = c0 < kCvTileW + 2 * pad, col1 = c0 + 1 < kCvTileW + 2 * pad;
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            sh.cc[q][lane] = 0;
-            sh.cc[q][lane + 64] = 0;
-        }
-        auto update = [&](int y, unsigned delta) {
-#pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int c = lane + 64 * k;
-                if (c >= cw) continue;
-                const unsigned u = cv_code(t, xl + c, y);
-                if (u == kNone) continue;
-                const unsigned b = (unsigned)sh.rank[u + kDiffs] - (unsigned)b0;
-                if (b < (unsigned)B) atomicAdd(&sh.cc[b >> 2][c], delta << (8 * (b & 3)));
-            }
+        for (int q = 0; q < Q; q++) *(uint2 *)&sh.cc[q][c0] = make_uint2(0, 0);
+        auto add = [&](unsigned raw, int c, unsigned delta) {
+            const unsigned u = cv_decode(t, raw);
+            if (u == kNone) return;
+            const unsigned b = (unsigned)sh.rank[u + kDiffs] - (unsigned)b0;
+            if (b < (unsigned)B) atomicAdd(&sh.cc[b >> 2][c], delta << (8 * (b & 3)));
         };
-        const int ya = t.y0 - pad < 0 ? 0 : t.y0 - pad, yb = t.y0 + pad < t.h ? t.y0 + pad : t.h;
-        for (int y = ya; y < yb; y++) update(y, 1u);
+        // the window rows above the first row, eight at a time
+        const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
+        for (int y = ya; y < yb; y += 8) {
+            unsigned v[16];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                v[2 * j] = y + j < yb && col0 ? cv_raw(t, xl, y + j) : 0u;
+                v[2 * j + 1] = y + j < yb && col1 ? cv_raw(t, xl + 1, y + j) : 0u;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                add(v[2 * j], c0, 1u);
+                add(v[2 * j + 1], c0 + 1, 1u);
+            }
+        }
+        // codes of the row: entering (y + pad), own (y), leaving (y - pad); loaded a row ahead
+        auto load = [&](int y, unsigned *v) {
+            v[0] = col0 ? cv_raw(t, xl, y + pad) : 0u;
+            v[1] = col1 ? cv_raw(t, xl + 1, y + pad) : 0u;
+            v[2] = cv_raw(t, t.x0 + lane, y);
+            v[3] = col0 ? cv_raw(t, xl, y - pad) : 0u;
+            v[4] = col1 ? cv_raw(t, xl + 1, y - pad) : 0u;
+        };
+        unsigned cur[5], nxt[5];
+        load(t.y0, cur);
         for (int r = 0; r < t.rows; r++) {
             const int y = t.y0 + r;
-            if (y + pad < t.h) update(y + pad, 1u);
-            const unsigned u = cv_code(t, t.x0 + lane, y);
+            if (r + 1 < t.rows) load(y + 1, nxt);
+            add(cur[0], c0, 1u);
+            add(cur[1], c0 + 1, 1u);
+            const unsigned u = cv_decode(t, cur[2]);
             const unsigned own = u == kNone ? (unsigned)kNone : sh.rank[u + kDiffs];
             const bool serve = own != kNone && (int)own >= lo && (int)own < hi;
             if (__any(serve)) {
+                // column counts of both columns, all number pairs at once, widened to 16 bits per number
+                uint32_t x[W2], e[W2];
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
-                    const uint32_t qa = sh.cc[q][lane], qb = sh.cc[q][lane + 64];
+                    const uint2 cq = *(const uint2 *)&sh.cc[q][c0];
 #pragma unroll
                     for (int hf = 0; hf < 2; hf++) {
-                        const uint32_t ra = qa >> (16 * hf), rb = qb >> (16 * hf);
-                        const uint32_t a = (ra & 0xFFu) | ((ra & 0xFF00u) << 8), b = (rb & 0xFFu) | ((rb & 0xFF00u) << 8);
-                        const uint32_t sa = wave_scan_incl(a), sb = wave_scan_incl(b);
-                        const uint32_t ta = (uint32_t)__builtin_amdgcn_readlane((int)sa, 63);
-                        const uint32_t tb = (uint32_t)__builtin_amdgcn_readlane((int)sb, 63);
-                        const uint32_t pa = sa - a, pb = ta + sb - b;  // P(lane), P(64 + lane)
-                        const uint32_t give = lane >= span ? pa : pb;  // what the lane span below this one needs
-                        uint32_t far = (uint32_t)__builtin_amdgcn_ds_bpermute(((lane + span) & 63) << 2, (int)give);
-                        if (lane + span == 128) far = ta + tb;
-                        sh.hh[2 * q + hf][lane] = far - pa;
+                        const uint32_t ra = cq.x >> (16 * hf), rb = cq.y >> (16 * hf);
+                        e[2 * q + hf] = (ra & 0xFFu) | ((ra & 0xFF00u) << 8);              // column c0
+                        x[2 * q + hf] = e[2 * q + hf] + ((rb & 0xFFu) | ((rb & 0xFF00u) << 8));  // both columns
                     }
                 }
-                float c = 0.0f;
-                if (serve) {
-                    auto count = [&](unsigned rk) -> unsigned {
-                        const unsigned b = rk - (unsigned)b0;
-                        return b < (unsigned)B ? (sh.hh[b >> 1][lane] >> (16 * (b & 1))) & 0xFFFFu : 0u;
-                    };
-                    const int value = (int)u + t.band_base + kDiffs;
-                    const unsigned p0 = count(own);
+                uint32_t s[W2];
 #pragma unroll
-                    for (int d = 0; d < kDiffs; d++) {
-                        if (value <= t.tvi[d] && value + d + 1 > t.vlt) {
-                            const unsigned p1 = count(sh.rank[u + kDiffs + d + 1]), p2 = count(sh.rank[u + kDiffs - d - 1]);
-                            const unsigned pm = p1 > p2 ? p1 : p2;
-                            const float val = __fmul_rn((float)((unsigned)t.weight[d] * p0 * pm), t.lut[pm + p0]);
-                            if (val > c) c = val;
-                        }
-                    }
-                    if (t.plane) t.plane[(size_t)y * t.w + t.x0 + lane] = c;
-                    if (c > 0.0f) sum_fx += fixed24(c);
+                for (int j = 0; j < W2; j++) s[j] = x[j];
+#define ES_SCAN_STEP(ctrl, rmask)                                                                        \
+    _Pragma("unroll") for (int j = 0; j < W2; j++) s[j] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s[j], ctrl, rmask, 0xF, false);
+                ES_SCAN_STEP(0x111, 0xF)
+                ES_SCAN_STEP(0x112, 0xF)
+                ES_SCAN_STEP(0x114, 0xF)
+                ES_SCAN_STEP(0x118, 0xF)
+                ES_SCAN_STEP(0x142, 0xA)
+                ES_SCAN_STEP(0x143, 0xC)
+#undef ES_SCAN_STEP
+                // P(c0) = s - x, P(c0 + 1) = P(c0) + count(c0), P(c0 + 2) = s (the next lane's P(c0) as well)
+#pragma unroll
+                for (int j = 0; j < W2; j++) {
+                    const uint32_t p = s[j] - x[j];
+                    *(uint2 *)&sh.pp[j][c0] = make_uint2(p, p + e[j]);
                 }
-                const bool pos = serve && c > 0.0f;
-                const unsigned long long m = __ballot(pos);
-                if (m) {
-                    unsigned base = 0;
-                    if (lane == 0) base = atomicAdd(t.count, (unsigned)__popcll(m));
-                    base = (unsigned)__builtin_amdgcn_readfirstlane((int)base);
-                    if (pos) t.list[base + __popcll(m & ((1ull << lane) - 1))] = c;
+#pragma unroll
+                for (int j = 0; j < W2; j++) sh.pp[j][c0 + 2] = s[j];
+                __syncthreads();  // the prefixes of all columns are in before any lane reads them
+                // the numbers of p0 and of the neighbours at +-(d + 1), and their window counts
+                const unsigned uu = serve ? u : 0u;
+                unsigned rk[2 * kDiffs + 1];
+                rk[kDiffs] = own;
+#pragma unroll
+                for (int d = 0; d < kDiffs; d++) {
+                    rk[kDiffs + d + 1] = sh.rank[uu + kDiffs + d + 1];
+                    rk[kDiffs - d - 1] = sh.rank[uu + kDiffs - d - 1];
                 }
+                unsigned cnt[2 * kDiffs + 1];
+#pragma unroll
+                for (int k = 0; k < 2 * kDiffs + 1; k++) {
+                    const unsigned b = rk[k] - (unsigned)b0, bb = b < (unsigned)B ? b : 0u;
+                    const uint32_t dlt = sh.pp[bb >> 1][lane + span] - sh.pp[bb >> 1][lane];
+                    cnt[k] = b < (unsigned)B ? (dlt >> (16 * (bb & 1))) & 0xFFFFu : 0u;
+                }
+                const int value = (int)uu + t.band_base + kDiffs;
+                const unsigned p0 = cnt[kDiffs];
+                float c = 0.0f;
+#pragma unroll
+                for (int d = 0; d < kDiffs; d++) {
+                    const unsigned p1 = cnt[kDiffs + d + 1], p2 = cnt[kDiffs - d - 1];
+                    const unsigned pm = p1 > p2 ? p1 : p2;
+                    // w * p0 * pm * RN(1 / (p0 + pm)): the reciprocal correctly rounded, as libvmaf's table holds it
+                    const float rcp = __fdiv_rn(1.0f, (float)(p0 + pm));
+                    const float val = __fmul_rn((float)((unsigned)t.weight[d] * p0 * pm), rcp);
+                    const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt && pm > 0;
+                    c = on && val > c ? val : c;
+                }
+                if (!serve) c = 0.0f;
+                if (serve && t.plane) t.plane[(size_t)y * t.w + t.x0 + lane] = c;
+                if (c > 0.0f) sum_fx += fixed24(c);
+                const unsigned long long m = __ballot(c > 0.0f);
+                if (c > 0.0f) t.seg[tcount + __popcll(m & ((1ull << lane) - 1))] = c;
+                tcount += (unsigned)__popcll(m);
+                __syncthreads();  // every lane has read the prefixes before the next row writes them
             }
-            if (y - pad >= 0) update(y - pad, ~0u);
+            add(cur[3], c0, ~0u);
+            add(cur[4], c0 + 1, ~0u);
+#pragma unroll
+            for (int k = 0; k < 5; k++) cur[k] = nxt[k];
         }
         if (hi == relevant) break;
         b0 = hi - kDiffs;
@@ -318,30 +369,47 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
         t.tvi[d] = K.tvi[d];
         t.weight[d] = K.weight[d];
     }
-    t.lut = pic.lut;
     t.plane = pic.cvals ? pic.cvals + pic.off[s] : nullptr;
-    t.list = pic.list + pic.off[s];
-    t.count = pic.counts + s;
-    const int lane = threadIdx.x, nv = t.band_size + 2 * kDiffs, cw = kCvTileW + 2 * t.pad;
-    __syncthreads();  // the previous tile's reads of hh and rank are done
-    uint8_t *served = (uint8_t *)&sh.hh[0][0], *occupied = served + kCvFlags;
+    t.seg = pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h;
+    unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
+    const int lane = threadIdx.x, nv = t.band_size + 2 * kDiffs;
+    const int c0 = 2 * lane, xl = t.x0 - t.pad + c0, cw = kCvTileW + 2 * t.pad;
+    __syncthreads();  // the previous tile's reads of pp and rank are done
+    uint8_t *served = (uint8_t *)&sh.pp[0][0], *occupied = served + kCvFlags;
     for (int i = lane; i < 2 * kCvFlags / 4; i += 64) ((uint32_t *)served)[i] = 0;
     __syncthreads();
     bool any = false;
-    for (int r = 0; r < t.rows; r++) {
-        const unsigned u = cv_code(t, t.x0 + lane, t.y0 + r);
-        if (u != kNone) {
-            served[u + kDiffs] = 1;
-            any = true;
+    for (int r = 0; r < t.rows; r += 8) {
+        unsigned v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = r + j < t.rows ? cv_raw(t, t.x0 + lane, t.y0 + r + j) : 0u;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const unsigned u = cv_decode(t, v[j]);
+            if (u != kNone) {
+                served[u + kDiffs] = 1;
+                any = true;
+            }
         }
     }
-    if (!__any(any)) return;  // most tiles: nothing masked inside the band
+    if (!__any(any)) {  // most tiles of most pictures: nothing masked inside the band
+        if (lane == 0) *seg_count = 0;
+        return;
+    }
     const int ya = max(t.y0 - t.pad, 0), yb = min(t.y0 + t.rows + t.pad, t.h);
-    for (int y = ya; y < yb; y++)
-        for (int c = lane; c < cw; c += 64) {
-            const unsigned u = cv_code(t, t.x0 - t.pad + c, y);
+    for (int y = ya; y < yb; y += 8) {
+        unsigned v[16];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            v[2 * j] = y + j < yb && c0 < cw ? cv_raw(t, xl, y + j) : 0u;
+            v[2 * j + 1] = y + j < yb && c0 + 1 < cw ? cv_raw(t, xl + 1, y + j) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const unsigned u = cv_decode(t, v[j]);
             if (u != kNone) occupied[u + kDiffs] = 1;
         }
+    }
     __syncthreads();
     int relevant = 0;
     for (int i0 = 0; i0 < nv; i0 += 64) {
@@ -354,13 +422,19 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
         relevant += __popcll(m);
     }
     __syncthreads();
-    if (relevant == 0) return;
     unsigned long long sum_fx = 0;
-    if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_fx);
-    else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_fx);
-    else cv_passes<32>(sh, t, relevant, sum_fx);
+    unsigned tcount = 0;
+    if (relevant > 0) {
+        if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_fx, tcount);
+        else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_fx, tcount);
+        else cv_passes<32>(sh, t, relevant, sum_fx, tcount);
+    }
     sum_fx = wave_sum(sum_fx);
-    if (lane == 0 && sum_fx) atomicAdd(pic.sums + s, sum_fx);
+    if (lane == 0) {
+        *seg_count = tcount;
+        if (tcount) atomicAdd(pic.counts + s, tcount);
+        if (sum_fx) atomicAdd(pic.sums + s, sum_fx);
+    }
 }
 
 __global__ __launch_bounds__(64) void cambi_cvalues_kernel(const CambiPic *pics, const int *tile_first, int nslots,
@@ -417,13 +491,17 @@ __global__ __launch_bounds__(256) void topk_pass_kernel(TopkJob *jobs, int pass)
     __syncthreads();
     const int shift = 18 - 9 * pass;
     const unsigned prefix = pass ? J.prefix : 0u;
-    for (unsigned i = blockIdx.x * 256u + tid; i < n; i += gridDim.x * 256u) {
-        const float c = J.list[i];
-        const unsigned key = __float_as_uint(c) - kKeyBase;
-        if ((key >> (shift + 9)) == (prefix >> (shift + 9))) {
-            const unsigned d = (key >> shift) & 511u;
-            atomicAdd(&hist[d], 1u);
-            atomicAdd(&hs[d], fixed24(c));
+    for (int sg = blockIdx.x; sg < J.nseg; sg += gridDim.x) {
+        const unsigned cnt = J.seg_count[sg];
+        const float *seg = J.list + (size_t)sg * J.seg_cap;
+        for (unsigned i = tid; i < cnt; i += 256) {
+            const float c = seg[i];
+            const unsigned key = __float_as_uint(c) - kKeyBase;
+            if ((key >> (shift + 9)) == (prefix >> (shift + 9))) {
+                const unsigned d = (key >> shift) & 511u;
+                atomicAdd(&hist[d], 1u);
+                atomicAdd(&hs[d], fixed24(c));
+            }
         }
     }
     __syncthreads();

@@ -2,7 +2,6 @@
 #include "host_common.hpp"
 
 #include <cmath>
-#include <map>
 
 namespace es {
 
@@ -50,22 +49,6 @@ ArgArena &arena()
 {
     static ArgArena *a = new ArgArena();  // lives until the process ends
     return *a;
-}
-
-const float *device_lut(int size)
-{
-    static std::mutex m;
-    static std::map<int, DevBuf *> luts;
-    std::lock_guard<std::mutex> lock(m);
-    auto it = luts.find(size);
-    if (it != luts.end()) return it->second->as<float>();
-    std::vector<float> h(size);
-    h[0] = 0.0f;
-    for (int i = 1; i < size; i++) h[i] = 1.0f / (float)i;  // libvmaf's reciprocal table: 1/i in single precision
-    DevBuf *b = new DevBuf((size_t)size * 4);
-    hip_check(hipMemcpy(b->p, h.data(), (size_t)size * 4, hipMemcpyHostToDevice), "reciprocal table");
-    luts[size] = b;
-    return b->as<float>();
 }
 
 // BT.1886 on the limited 10-bit range (libvmaf's luminance functions, called at src/cambi.cpp:41-53): white 300
@@ -138,8 +121,6 @@ CambiInfo cambi_info(int w, int h, int depth, int window_base)
         c.k[s] = std::min(std::max(k, 1LL), c.n[s]);
         c.off[s + 1] = c.off[s] + c.n[s];
     }
-    const int span = 2 * c.pad + 1;
-    c.lut_size = span * span + 1;
     return c;
 }
 

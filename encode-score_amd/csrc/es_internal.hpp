@@ -44,23 +44,28 @@ struct PlaneIn {
 };
 
 // One picture for the CAMBI kernels.  The packed scale planes hold value | mask << 15 per pixel, scale s at
-// planes + off[s]; the optional full c-value planes and the compact lists of positive c-values use the same offsets.
+// planes + off[s] (and the optional full c-value planes at cvals + off[s]).  The positive c-values of a scale go to
+// per-tile segments: tile j of scale s (64 columns by the launch's tile height) writes its values from
+// list + loff[s] + j * 64 * tile_h and their number to seg_count[toff[s] + j], so no tile waits on another.
 struct CambiPic {
     PlaneIn in;
     int sw[kScales], sh[kScales];
-    long long off[kScales];
+    long long off[kScales], loff[kScales];
+    int toff[kScales];
     uint16_t *planes;
     float *cvals;                 // full c-value planes or nullptr
-    float *list;                  // positive c-values, scale s from list + off[s]
+    float *list;                  // per-tile segments of positive c-values
+    unsigned int *seg_count;      // positive c-values per tile
     unsigned long long *sums;     // [kScales] exact sums, units of 2^-24
-    unsigned int *counts;         // [kScales] positive c-values (the list lengths)
+    unsigned int *counts;         // [kScales] positive c-values of the scale
     const CambiConsts *consts;    // device pointer
-    const float *lut;             // reciprocal table
 };
 
 // Radix-select state of one (picture, scale) top-k: the k-th largest positive c-value when more than k are positive.
 struct TopkJob {
-    const float *list;
+    const float *list;             // the scale's segments, seg_cap values apart
+    const unsigned int *seg_count; // values per segment
+    int nseg, seg_cap;
     const unsigned int *count;
     unsigned long long k;
     unsigned int *hist;            // [512] counts of the current digit
@@ -69,6 +74,23 @@ struct TopkJob {
     unsigned int prefix, remaining;
     unsigned long long above, above_fx;
 };
+
+// The per-tile segment layout of a picture's c-value lists for tiles of tile_h rows: list offsets and tile offsets
+// per scale; returns the capacity in values and the number of tiles.
+inline long long cambi_list_layout(const int *sw, const int *sh, int tile_h, long long *loff, int *toff, int *tiles)
+{
+    long long cap = 0;
+    int t = 0;
+    for (int s = 0; s < kScales; s++) {
+        const int n = ((sw[s] + kCvTileW - 1) / kCvTileW) * ((sh[s] + tile_h - 1) / tile_h);
+        loff[s] = cap;
+        toff[s] = t;
+        cap += (long long)n * kCvTileW * tile_h;
+        t += n;
+    }
+    if (tiles) *tiles = t;
+    return cap;
+}
 
 // Host-side launch wrappers (cambi_kernels.hip).
 void launch_cambi_base(const CambiPic *pics, const int *tile_first, int npics, int total_tiles, hipStream_t st);

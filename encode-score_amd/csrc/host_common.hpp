@@ -138,9 +138,7 @@ struct CambiInfo {
     int tvi[kDiffs], weight[kDiffs];
     int sw[kScales], sh[kScales];
     long long n[kScales], k[kScales], off[kScales + 1];
-    int lut_size;
 };
 CambiInfo cambi_info(int w, int h, int depth, int window_base);
-const float *device_lut(int size);  // the reciprocal table, shared per size
 
 }  // namespace es

@@ -283,7 +283,7 @@ public:
         tile_h = c.tile_h > 0 ? c.tile_h : 32;
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "device attributes");
-        cv_blocks = std::max(1, cus) * 8;
+        cv_blocks = std::max(1, cus) * 16;
         const int C = (int)crops.size(), E = (int)enc.size();
         roles = C + E;
         P = B * roles;
@@ -306,10 +306,23 @@ public:
         }
         hip_check(hipMemcpy(consts.p, hk.data(), consts.bytes, hipMemcpyHostToDevice), "cambi constants");
         slot_off.resize(P + 1);
-        slot_off[0] = 0;
-        for (int p = 0; p < P; p++) slot_off[p + 1] = slot_off[p] + (size_t)crops[crop_of_role(p % roles)].ci.off[kScales];
+        list_off.resize(P + 1);
+        seg_off.resize(P + 1);
+        slot_off[0] = list_off[0] = seg_off[0] = 0;
+        lay.resize(C);
+        for (int k = 0; k < C; k++) {
+            const CambiInfo &ci = crops[k].ci;
+            lay[k].cap = cambi_list_layout(ci.sw, ci.sh, tile_h, lay[k].loff, lay[k].toff, &lay[k].tiles);
+        }
+        for (int p = 0; p < P; p++) {
+            const int k = crop_of_role(p % roles);
+            slot_off[p + 1] = slot_off[p] + (size_t)crops[k].ci.off[kScales];
+            list_off[p + 1] = list_off[p] + (size_t)lay[k].cap;
+            seg_off[p + 1] = seg_off[p] + (size_t)lay[k].tiles;
+        }
         planes.alloc(slot_off[P] * 2);
-        list.alloc(slot_off[P] * 4);
+        list.alloc(list_off[P] * 4);
+        segs.alloc(seg_off[P] * 4);
         sums.alloc((size_t)P * kScales * 8);
         counts.alloc((size_t)P * kScales * 4 + 64);  // + the work counter
         thist.alloc((size_t)P * kScales * 512 * 4);
@@ -320,11 +333,15 @@ public:
         hip_check(hipMemset(tticket.p, 0, tticket.bytes), "clearing");
         std::vector<TopkJob> jobs((size_t)P * kScales);
         for (int p = 0; p < P; p++) {
-            const CambiInfo &ci = crops[crop_of_role(p % roles)].ci;
+            const int kc = crop_of_role(p % roles);
+            const CambiInfo &ci = crops[kc].ci;
             for (int s = 0; s < kScales; s++) {
                 TopkJob &j = jobs[(size_t)p * kScales + s];
                 j = TopkJob{};
-                j.list = list.as<float>() + slot_off[p] + ci.off[s];
+                j.list = list.as<float>() + list_off[p] + lay[kc].loff[s];
+                j.seg_count = segs.as<unsigned>() + seg_off[p] + lay[kc].toff[s];
+                j.nseg = cvalue_tiles(ci.sw[s], ci.sh[s], tile_h);
+                j.seg_cap = kCvTileW * tile_h;
                 j.count = counts.as<unsigned>() + (size_t)p * kScales + s;
                 j.k = (unsigned long long)ci.k[s];
                 j.hist = thist.as<unsigned>() + ((size_t)p * kScales + s) * 512;
@@ -468,8 +485,13 @@ private:
 
     int B, R, P, roles, tile_h, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
     std::vector<long long> max_px;
-    std::vector<size_t> slot_off;
-    DevBuf consts, planes, list, sums, counts, thist, thsum, tticket, topk, scores;
+    struct Layout {
+        long long loff[kScales], cap;
+        int toff[kScales], tiles;
+    };
+    std::vector<Layout> lay;
+    std::vector<size_t> slot_off, list_off, seg_off;
+    DevBuf consts, planes, list, segs, sums, counts, thist, thsum, tticket, topk, scores;
     std::vector<CropDev> cd;
     std::vector<EncDev> ed;
     DevBuf tot_int, tot_or, tot_last, quant, scratch, desc;
@@ -527,6 +549,8 @@ private:
                     pc.sw[s] = on ? ci.sw[s] : 0;
                     pc.sh[s] = on ? ci.sh[s] : 0;
                     pc.off[s] = ci.off[s];
+                    pc.loff[s] = lay[c].loff[s];
+                    pc.toff[s] = lay[c].toff[s];
                 }
                 if (on) {
                     pc.in = r < C ? src_plane(src[i], r) : enc_plane(encp[(size_t)i * E + (r - C)], r - C);
@@ -537,11 +561,11 @@ private:
                 }
                 pc.planes = planes.as<uint16_t>() + slot_off[pidx];
                 pc.cvals = nullptr;
-                pc.list = list.as<float>() + slot_off[pidx];
+                pc.list = list.as<float>() + list_off[pidx];
+                pc.seg_count = segs.as<unsigned>() + seg_off[pidx];
                 pc.sums = sums.as<unsigned long long>() + (size_t)pidx * kScales;
                 pc.counts = counts.as<unsigned>() + (size_t)pidx * kScales;
                 pc.consts = consts.as<CambiConsts>() + c;
-                pc.lut = device_lut(ci.lut_size);
             }
         }
         first[P * kScales] = t;
