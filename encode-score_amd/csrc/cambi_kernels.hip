@@ -38,82 +38,81 @@ __device__ __forceinline__ uint16_t mode3(uint16_t a, uint16_t b, uint16_t c)
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Scale 0: spatial mask + mode filter, one 64 x 32 tile per block of 256 threads.
+// Scale 0: spatial mask + mode filter.  One wave walks down a strip of 64 columns and kBaseRows rows.  Per row each
+// lane reads the pixels 32 left and 32 right of its output column (128 columns in all, the strip and its margins);
+// two ballots give the flat bits of all of them, and the lane counts the 7 around its column with one popcount of a
+// 128-bit window; a running sum over the last 7 rows gives the 7x7 count.  The horizontal 3-modes of the last three
+// rows (neighbours by lane shuffles) give the vertical mode.  The mask lags the row read by 3 rows and the mode by 1,
+// so the packed value is written 3 rows back.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int kBaseW = 64, kBaseH = 32;
+constexpr int kBaseW = 64, kBaseRows = 64;
 
-int base_tiles(int w, int h) { return ((w + kBaseW - 1) / kBaseW) * ((h + kBaseH - 1) / kBaseH); }
+int base_tiles(int w, int h) { return ((w + kBaseW - 1) / kBaseW) * ((h + kBaseRows - 1) / kBaseRows); }
 
 __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
 {
-    __shared__ uint16_t v[kBaseH + 8][kBaseW + 8];  // the picture around the tile, 4 pixels of margin
-    __shared__ uint8_t flat[kBaseH + 6][kBaseW + 6];
-    __shared__ uint8_t hs[kBaseH + 6][kBaseW];      // flat pixels among the 7 of each row around a column
     const CambiPic &pic = pics[blockIdx.y];
     const int w = pic.sw[0], h = pic.sh[0];
+    const int lane = threadIdx.x & 63, strip = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int across = (w + kBaseW - 1) / kBaseW;
-    if ((int)blockIdx.x >= across * ((h + kBaseH - 1) / kBaseH)) return;
-    const int x0 = (blockIdx.x % across) * kBaseW, y0 = (blockIdx.x / across) * kBaseH;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < (kBaseH + 8) * (kBaseW + 8); i += 256) {
-        const int r = i / (kBaseW + 8), c = i % (kBaseW + 8);
-        const int x = x0 - 4 + c, y = y0 - 4 + r;
-        v[r][c] = (x >= 0 && x < w && y >= 0 && y < h) ? (uint16_t)base_value(pic.in, w, h, x, y) : kNone;
-    }
-    __syncthreads();
-    // flat: equal to the right and lower neighbours (the last column and row count as equal); never outside
-    for (int i = tid; i < (kBaseH + 6) * (kBaseW + 6); i += 256) {
-        const int r = i / (kBaseW + 6), c = i % (kBaseW + 6);
-        const int x = x0 - 3 + c, y = y0 - 3 + r;
-        uint8_t f = 0;
-        if (x >= 0 && x < w && y >= 0 && y < h) {
-            const uint16_t a = v[r + 1][c + 1];
-            f = (uint8_t)((x == w - 1 || a == v[r + 1][c + 2]) && (y == h - 1 || a == v[r + 2][c + 1]));
-        }
-        flat[r][c] = f;
-    }
-    __syncthreads();
-    for (int i = tid; i < (kBaseH + 6) * kBaseW; i += 256) {
-        const int r = i / kBaseW, c = i % kBaseW;
-        int s = 0;
-#pragma unroll
-        for (int d = 0; d < 7; d++) s += flat[r][c + d];
-        hs[r][c] = (uint8_t)s;
-    }
-    __syncthreads();
+    if (strip >= across * ((h + kBaseRows - 1) / kBaseRows)) return;  // uniform over the wave
+    const int x0 = (strip % across) * kBaseW, y0 = (strip / across) * kBaseRows;
+    const int y1 = min(y0 + kBaseRows, h);
+    const int xa = x0 - 32 + lane, xb = x0 + 32 + lane, x = x0 + lane;
     const int index = pic.consts->mask_index;
-    const int c = tid & 63, g = tid >> 6, x = x0 + c;
-    if (x >= w) return;
-    uint16_t *out = pic.planes;
-#pragma unroll 2
-    for (int j = 0; j < kBaseH / 4; j++) {
-        const int r = g * (kBaseH / 4) + j, y = y0 + r;
-        if (y >= h) break;
-        int count = 0;
+    auto at = [&](int xx, int yy) -> int {
+        return (xx >= 0 && xx < w && yy >= 0 && yy < h) ? (int)base_value(pic.in, w, h, xx, yy) : -1;
+    };
+    // rows r - 3 .. r + 3 of the mask window end at r + 3; the flat bits of row r need row r + 1
+    int ca = at(xa, y0 - 3), cb = at(xb, y0 - 3), cc = at(x, y0 - 3);
+    int last[7] = {0, 0, 0, 0, 0, 0, 0}, sum = 0;
+    int hm[3] = {0, 0, 0}, cv[3] = {0, 0, 0};  // horizontal modes and own values of rows r - 2 .. r
+    int md[3] = {0, 0, 0};                      // modes of rows r - 3 .. r - 1 (written 3 rows back)
+    for (int r = y0 - 3; r < y1 + 3; r++) {
+        const int na = at(xa, r + 1), nb = at(xb, r + 1), nc = at(x, r + 1);
+        // flat bits of row r over columns x0 - 32 .. x0 + 95
+        const int ra0 = __shfl_down(ca, 1), rb = __shfl_down(cb, 1), b0 = __shfl(cb, 0);
+        const int ra = lane == 63 ? b0 : ra0;
+        const bool fa = ca >= 0 && (xa == w - 1 || ca == ra) && (r == h - 1 || ca == na);
+        const bool fb = cb >= 0 && (xb == w - 1 || cb == rb) && (r == h - 1 || cb == nb);
+        const unsigned long long lo = __ballot(fa), hi = __ballot(fb);
+        // columns x - 3 .. x + 3 are bits lane + 29 .. lane + 35 of hi:lo
+        const int sft = lane + 29;
+        const unsigned long long win = sft < 64 ? (lo >> sft) | (hi << (64 - sft)) : hi >> (sft - 64);
+        const int count = __popcll(win & 0x7Full);
+        sum += count - last[0];
 #pragma unroll
-        for (int d = 0; d < 7; d++) count += hs[r + d][c];
-        const unsigned mask = count > index;
-        const int vr = r + 4, vc = c + 4;
-        uint16_t m;
-        if (y == 0 || y == h - 1) {
-            m = v[vr][vc];
-        } else {
-            uint16_t t[3];
-#pragma unroll
-            for (int q = 0; q < 3; q++) {
-                const int row = vr - 1 + q;
-                t[q] = (x == 0 || x == w - 1) ? v[row][vc] : mode3(v[row][vc - 1], v[row][vc], v[row][vc + 1]);
-            }
-            m = mode3(t[0], t[1], t[2]);
-        }
-        out[(size_t)y * w + x] = (uint16_t)(m | (mask << 15));
+        for (int k = 0; k < 6; k++) last[k] = last[k + 1];
+        last[6] = count;
+        // horizontal mode of row r at column x; the first and last columns keep their own value
+        // every lane takes part in every shuffle; the edge lanes then pick the margin columns' values
+        const int left = __shfl_up(cc, 1), right = __shfl_down(cc, 1), a31 = __shfl(ca, 31), b32 = __shfl(cb, 32);
+        const int lft = lane == 0 ? a31 : left, rgt = lane == 63 ? b32 : right;
+        const int h3 = (x == 0 || x >= w - 1) ? cc : (int)mode3((uint16_t)lft, (uint16_t)cc, (uint16_t)rgt);
+        hm[0] = hm[1];
+        hm[1] = hm[2];
+        hm[2] = h3;
+        cv[0] = cv[1];
+        cv[1] = cv[2];
+        cv[2] = cc;
+        // the mode of row r - 1: its first and last rows are copied
+        const int ym = r - 1;
+        const int m = (ym == 0 || ym == h - 1) ? cv[1] : (int)mode3((uint16_t)hm[0], (uint16_t)hm[1], (uint16_t)hm[2]);
+        md[0] = md[1];
+        md[1] = md[2];
+        md[2] = m;
+        const int y = r - 3;  // the row whose 7x7 count is complete; its mode is md[0]
+        if (y >= y0 && y < y1 && x < w) pic.planes[(size_t)y * w + x] = (uint16_t)(md[0] | ((sum > index ? 1 : 0) << 15));
+        ca = na;
+        cb = nb;
+        cc = nc;
     }
 }
 
 void launch_cambi_base(const CambiPic *pics, const int * /*tile_first*/, int npics, int max_tiles, hipStream_t st)
 {
     if (npics <= 0) return;
-    hipLaunchKernelGGL(cambi_base_kernel, dim3(max_tiles, npics), dim3(256), 0, st, pics);
+    hipLaunchKernelGGL(cambi_base_kernel, dim3((max_tiles + 3) / 4, npics), dim3(256), 0, st, pics);
     hip_check(hipGetLastError(), "cambi base");
 }
 

@@ -26,7 +26,24 @@ struct PersistJob {
     long long *out;
 };
 
+// Lasting texture of the batched form, once per crop and round: the centres centre0 .. centre0 + ncentres - 1 in
+// order, with the source's SA and SB carried from one centre to the next as exact running sums
+// (SA(c) = SB(c - 1) - m(c) + m(c + 6), SB(c) = SA(c - 1) - m(c - 7) + m(c - 1); computed in full at the first centre
+// of the run), and each of the crop's encodes' projections on ST.  Centre c is round slot c + 6 - f0.
+constexpr int kMaxCropEncodes = 16;
+struct LastingJob {
+    const int32_t *ring;              // the crop's source mid planes, `slots` of them, slot of frame f at f % slots
+    const int32_t *emid[kMaxCropEncodes];  // the encodes' mid planes, same slots
+    long long *penc[kMaxCropEncodes];     // per encode: [slot][window] M_d(c) ST
+    int last_centre[kMaxCropEncodes];     // per encode: its last centre with lasting texture
+    int nenc, slots, w, h;
+    int32_t *sa, *sb;                 // running sums of the previous centre, per pixel
+    long long *psrc;                  // [slot][window][4]: M_r(c) ST, SA SB, SA^2, SB^2
+    int centre0, ncentres, slot0;
+};
+
 void launch_window_stats(const WsPair *pairs, int npairs, int max_w, int max_h, int win, bool fp64, hipStream_t st);
+void launch_lasting(const LastingJob *jobs, int njobs, int max_w, int max_h, int win, hipStream_t st);
 void launch_persist(const PersistJob *jobs, int njobs, int max_w, int max_h, int win, int kind, hipStream_t st);
 
 // Device collect of one (encode, frame): structure threshold, mask, histogram entries and frame totals; and of the

@@ -217,57 +217,118 @@ __global__ __launch_bounds__(256) void persist_plain_kernel(const PersistJob *jo
     }
 }
 
-// Batched form, source side, once per (crop, centre): ST plane and [M_r(c) ST, SA SB, SA^2, SB^2] per window.
+// Batched form: one wave per 16 x 16 patch of the area the windows cover, 4 pixels of a row per lane; the patch sums
+// of a centre go to the window directly (16 x 16 windows) or by 64-bit atomic adds into cleared outputs (larger ones).
 template <int WIN>
-__global__ __launch_bounds__(256) void persist_src_kernel(const PersistJob *jobs)
+__global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
 {
-    __shared__ long long part[4][4];
-    const PersistJob &J = jobs[blockIdx.z];
-    const int wx = blockIdx.x, wy = blockIdx.y, ww = J.w / WIN;
-    if (wx >= ww || wy >= J.h / WIN) return;
-    long long v[4] = {0, 0, 0, 0};
-    for (int i = threadIdx.x; i < WIN * WIN; i += 256) {
-        const size_t k = (size_t)(wy * WIN + i / WIN) * J.w + wx * WIN + i % WIN;
-        const int a = J.ring[0][k] + J.ring[2][k] + J.ring[4][k] + J.ring[8][k] + J.ring[10][k] + J.ring[12][k];
-        const int b = J.ring[1][k] + J.ring[3][k] + J.ring[5][k] + J.ring[7][k] + J.ring[9][k] + J.ring[11][k];
-        const int t = a + b;
-        J.st[k] = t;
-        v[0] += (long long)J.ring[6][k] * t;
-        v[1] += (long long)a * b;
-        v[2] += (long long)a * a;
-        v[3] += (long long)b * b;
+    const LastingJob &J = jobs[blockIdx.z];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ww = J.w / WIN, wh = J.h / WIN, pw = ww * (WIN / 16), ph = wh * (WIN / 16);
+    const int patch = blockIdx.x * 4 + wave;
+    if (J.ncentres <= 0 || patch >= pw * ph) return;  // uniform over the wave
+    const int px = patch % pw, py = patch / pw;
+    const int x = px * 16 + (lane & 3) * 4, y = py * 16 + (lane >> 2);
+    const size_t n = (size_t)J.w * J.h, i0 = (size_t)y * J.w + x;
+    const int window = (py * 16 / WIN) * ww + px * 16 / WIN;
+    auto plane = [&](int f) { return J.ring + (size_t)(f % J.slots) * n + i0; };
+    int sa[4], sb[4];
+    const int c0 = J.centre0;
+    if (c0 > kRadius) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            sa[k] = J.sa[i0 + k];
+            sb[k] = J.sb[i0 + k];
+        }
     }
-    block_sums(v, part);
-    if (threadIdx.x == 0) {
-        long long *o = J.out + ((size_t)wy * ww + wx) * 4;
-        for (int k = 0; k < 4; k++) o[k] = v[k];
+    for (int ci = 0; ci < J.ncentres; ci++) {
+        const int c = c0 + ci;
+        int m0[4];
+        if (c == kRadius) {  // the first centre of the run: the sums in full
+            const int32_t *p[13];
+#pragma unroll
+            for (int o = 0; o < 13; o++) p[o] = plane(c + o - kRadius);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                sa[k] = p[0][k] + p[2][k] + p[4][k] + p[8][k] + p[10][k] + p[12][k];
+                sb[k] = p[1][k] + p[3][k] + p[5][k] + p[7][k] + p[9][k] + p[11][k];
+                m0[k] = p[6][k];
+            }
+        } else {
+            const int32_t *pc = plane(c), *p6 = plane(c + 6), *p7 = plane(c - 7), *p1 = plane(c - 1);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                m0[k] = pc[k];
+                const int a = sb[k] - pc[k] + p6[k];
+                const int b = sa[k] - p7[k] + p1[k];
+                sa[k] = a;
+                sb[k] = b;
+            }
+        }
+        long long v[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long st = (long long)sa[k] + sb[k];
+            v[0] += m0[k] * st;
+            v[1] += (long long)sa[k] * sb[k];
+            v[2] += (long long)sa[k] * sa[k];
+            v[3] += (long long)sb[k] * sb[k];
+        }
+        const int slot = J.slot0 + ci;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const long long t = reduce_lanes(v[k], 64);
+            if (lane == 0) {
+                long long *o = J.psrc + ((size_t)slot * ww * wh + window) * 4 + k;
+                if (WIN == 16) *o = t;
+                else atomicAdd((unsigned long long *)o, (unsigned long long)t);
+            }
+        }
+        for (int e = 0; e < J.nenc; e++) {
+            if (c > J.last_centre[e]) continue;
+            const int32_t *pe = J.emid[e] + (size_t)(c % J.slots) * n + i0;
+            long long u = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) u += (long long)pe[k] * ((long long)sa[k] + sb[k]);
+            u = reduce_lanes(u, 64);
+            if (lane == 0) {
+                long long *o = J.penc[e] + (size_t)slot * ww * wh + window;
+                if (WIN == 16) *o = u;
+                else atomicAdd((unsigned long long *)o, (unsigned long long)u);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        J.sa[i0 + k] = sa[k];
+        J.sb[i0 + k] = sb[k];
     }
 }
 
-// Batched form, encode side, per (encode, centre): [M_d(c) ST] per window.
 template <int WIN>
-__global__ __launch_bounds__(256) void persist_enc_kernel(const PersistJob *jobs)
+static void launch_lasting_t(const LastingJob *jobs, int njobs, int max_w, int max_h, hipStream_t st)
 {
-    __shared__ long long part[1][4];
-    const PersistJob &J = jobs[blockIdx.z];
-    const int wx = blockIdx.x, wy = blockIdx.y, ww = J.w / WIN;
-    if (wx >= ww || wy >= J.h / WIN) return;
-    long long v[1] = {0};
-    for (int i = threadIdx.x; i < WIN * WIN; i += 256) {
-        const size_t k = (size_t)(wy * WIN + i / WIN) * J.w + wx * WIN + i % WIN;
-        v[0] += (long long)J.mid_d[k] * J.st[k];
-    }
-    block_sums(v, part);
-    if (threadIdx.x == 0) J.out[(size_t)wy * ww + wx] = v[0];
+    const int patches = (max_w / WIN) * (WIN / 16) * (max_h / WIN) * (WIN / 16);
+    if (patches <= 0) return;
+    hipLaunchKernelGGL(lasting_kernel<WIN>, dim3((patches + 3) / 4, 1, njobs), dim3(256), 0, st, jobs);
+}
+
+void launch_lasting(const LastingJob *jobs, int njobs, int max_w, int max_h, int win, hipStream_t st)
+{
+    if (njobs <= 0) return;
+    if (win == 16) launch_lasting_t<16>(jobs, njobs, max_w, max_h, st);
+    else if (win == 32) launch_lasting_t<32>(jobs, njobs, max_w, max_h, st);
+    else if (win == 64) launch_lasting_t<64>(jobs, njobs, max_w, max_h, st);
+    else throw Error(1, "window size must be 16, 32 or 64");
+    hip_check(hipGetLastError(), "lasting texture");
 }
 
 template <int WIN>
 static void launch_persist_t(const PersistJob *jobs, int njobs, int max_w, int max_h, int kind, hipStream_t st)
 {
     const dim3 grid(max_w / WIN, max_h / WIN, njobs);
-    if (kind == 0) hipLaunchKernelGGL(persist_plain_kernel<WIN>, grid, dim3(256), 0, st, jobs);
-    else if (kind == 1) hipLaunchKernelGGL(persist_src_kernel<WIN>, grid, dim3(256), 0, st, jobs);
-    else hipLaunchKernelGGL(persist_enc_kernel<WIN>, grid, dim3(256), 0, st, jobs);
+    (void)kind;
+    hipLaunchKernelGGL(persist_plain_kernel<WIN>, grid, dim3(256), 0, st, jobs);
 }
 
 void launch_persist(const PersistJob *jobs, int njobs, int max_w, int max_h, int win, int kind, hipStream_t st)

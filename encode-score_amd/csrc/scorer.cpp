@@ -367,8 +367,10 @@ public:
             const size_t n = (size_t)crops[k].w * crops[k].h;
             d.ring.alloc((size_t)R * n * 4);
             hip_check(hipMemset(d.ring.p, 0, d.ring.bytes), "clearing");
-            d.st.alloc((size_t)B * n * 4);
-            d.psrc.alloc((size_t)B * crops[k].windows * 4 * 8);
+            d.sa.alloc(n * 4);
+            d.sb.alloc(n * 4);
+            d.psrc = lout_words;
+            lout_words += (size_t)B * crops[k].windows * 4;
             cd.push_back(std::move(d));
         }
         for (int e = 0; e < E; e++) {
@@ -380,11 +382,15 @@ public:
             d.masks.alloc((size_t)R * cr.windows);
             hip_check(hipMemset(d.masks.p, 0, d.masks.bytes), "clearing");
             d.sums.alloc((size_t)B * cr.windows * 12 * 8);
-            d.penc.alloc((size_t)B * cr.windows * 8);
+            d.penc = lout_words;
+            lout_words += (size_t)B * cr.windows;
             d.hist.alloc(std::max<size_t>(1, enc[e].seg.size()) * 6 * kHistBins * 8);
             hip_check(hipMemset(d.hist.p, 0, d.hist.bytes), "clearing");
             ed.push_back(std::move(d));
         }
+        for (int e = 0; e < E; e++) require(std::count_if(enc.begin(), enc.end(), [&](const EncodeState &x) { return x.crop == enc[e].crop; }) <= kMaxCropEncodes,
+                                            "too many encodes of one crop");
+        lout.alloc(lout_words * 8);
         tot_int.alloc((size_t)B * E * 12 * 8);
         tot_or.alloc((size_t)B * E * 4 * 8);
         tot_last.alloc((size_t)B * E * 3 * 8);
@@ -392,8 +398,7 @@ public:
         scratch.alloc((size_t)B * E * max_windows * 8);
         // descriptors: device arrays, staged from pinned host memory per round (two staging buffers)
         desc_bytes = align(P * sizeof(CambiPic)) + align(((size_t)P * kScales + 1) * 4) + align((size_t)B * E * sizeof(WsPair)) +
-                     align((size_t)B * C * sizeof(PersistJob)) + align((size_t)B * E * sizeof(PersistJob)) +
-                     align((size_t)B * E * sizeof(CollectJob));
+                     align((size_t)C * sizeof(LastingJob)) + align((size_t)B * E * sizeof(CollectJob));
         desc.alloc(desc_bytes);
         res_bytes = align((size_t)P * 8) + align((size_t)B * E * 12 * 8) + align((size_t)B * E * 4 * 8) +
                     align((size_t)B * E * 3 * 8) + align((size_t)B * E * 8);
@@ -473,10 +478,12 @@ private:
     int crop_of_role(int r) const { return r < (int)crops.size() ? r : enc[r - crops.size()].crop; }
 
     struct CropDev {
-        DevBuf ring, st, psrc;
+        DevBuf ring, sa, sb;
+        size_t psrc = 0;  // offset in lout: [slot][window][4]
     };
     struct EncDev {
-        DevBuf mid, masks, sums, penc, hist;
+        DevBuf mid, masks, sums, hist;
+        size_t penc = 0;  // offset in lout: [slot][window]
     };
     struct Pending {
         long f0 = 0;
@@ -494,7 +501,8 @@ private:
     DevBuf consts, planes, list, segs, sums, counts, thist, thsum, tticket, topk, scores;
     std::vector<CropDev> cd;
     std::vector<EncDev> ed;
-    DevBuf tot_int, tot_or, tot_last, quant, scratch, desc;
+    DevBuf tot_int, tot_or, tot_last, quant, scratch, desc, lout;
+    size_t lout_words = 0;
     size_t desc_bytes, res_bytes;
     HostBuf stage_buf[2], res[2];
     hipEvent_t done[2] = {}, joined = nullptr;
@@ -514,9 +522,8 @@ private:
     size_t o_pics() const { return 0; }
     size_t o_first() const { return align((size_t)P * sizeof(CambiPic)); }
     size_t o_ws() const { return o_first() + align(((size_t)P * kScales + 1) * 4); }
-    size_t o_psrc() const { return o_ws() + align((size_t)B * enc.size() * sizeof(WsPair)); }
-    size_t o_penc() const { return o_psrc() + align((size_t)B * crops.size() * sizeof(PersistJob)); }
-    size_t o_col() const { return o_penc() + align((size_t)B * enc.size() * sizeof(PersistJob)); }
+    size_t o_last() const { return o_ws() + align((size_t)B * enc.size() * sizeof(WsPair)); }
+    size_t o_col() const { return o_last() + align((size_t)crops.size() * sizeof(LastingJob)); }
     size_t r_scores() const { return 0; }
     size_t r_ti() const { return align((size_t)P * 8); }
     size_t r_to() const { return r_ti() + align((size_t)B * enc.size() * 12 * 8); }
@@ -531,8 +538,7 @@ private:
         CambiPic *pics = (CambiPic *)(h + o_pics());
         int *first = (int *)(h + o_first());
         WsPair *ws = (WsPair *)(h + o_ws());
-        PersistJob *psj = (PersistJob *)(h + o_psrc());
-        PersistJob *pej = (PersistJob *)(h + o_penc());
+        LastingJob *lj = (LastingJob *)(h + o_last());
         CollectJob *cj = (CollectJob *)(h + o_col());
         int t = 0;
         for (int i = 0; i < B; i++) {
@@ -579,7 +585,6 @@ private:
                 const bool on = i < n && active(e, f);
                 WsPair &w = ws[(size_t)i * E + e];
                 CollectJob &j = cj[(size_t)i * E + e];
-                PersistJob &pe = pej[(size_t)i * E + e];
                 if (!on) continue;
                 w.r = src_plane(src[i], c);
                 w.d = enc_plane(encp[(size_t)i * E + e], e);
@@ -600,30 +605,45 @@ private:
                 j.tot_lasting = tot_last.as<long long>() + ((size_t)i * E + e) * 3;
                 if (f >= 2 * kRadius) {
                     const long centre = f - kRadius;
-                    pe.mid_d = ed[e].mid.as<int32_t>() + (size_t)(centre % R) * np;
-                    pe.st = cd[c].st.as<int32_t>() + (size_t)i * np;
-                    pe.w = cr.w;
-                    pe.h = cr.h;
-                    pe.out = ed[e].penc.as<long long>() + (size_t)i * cr.windows;
-                    j.psrc = cd[c].psrc.as<long long>() + (size_t)i * cr.windows * 4;
-                    j.penc = pe.out;
+                    j.psrc = lout.as<long long>() + cd[c].psrc + (size_t)i * cr.windows * 4;
+                    j.penc = lout.as<long long>() + ed[e].penc + (size_t)i * cr.windows;
                     j.cmask = ed[e].masks.as<uint8_t>() + (size_t)(centre % R) * cr.windows;
                     j.lhist = ed[e].hist.as<unsigned long long>() + (size_t)(centre / cfg.per_segment) * 6 * kHistBins;
                     j.lasting_plain = 0;
                 }
             }
-            for (int c = 0; c < C; c++) {
-                PersistJob &ps = psj[(size_t)i * C + c];
-                if (!(i < n && f >= 2 * kRadius && crop_active(c, f))) continue;
-                const Crop &cr = crops[c];
-                const size_t np = (size_t)cr.w * cr.h;
-                const long centre = f - kRadius;
-                for (int o = -kRadius; o <= kRadius; o++) ps.ring[o + kRadius] = cd[c].ring.as<int32_t>() + (size_t)((centre + o) % R) * np;
-                ps.st = cd[c].st.as<int32_t>() + (size_t)i * np;
-                ps.w = cr.w;
-                ps.h = cr.h;
-                ps.out = cd[c].psrc.as<long long>() + (size_t)i * cr.windows * 4;
+        }
+        // lasting texture per crop: the centres of the round's frames from frame 12 on while the crop is active
+        for (int c = 0; c < C; c++) {
+            LastingJob &L = lj[c];
+            const Crop &cr = crops[c];
+            const size_t np = (size_t)cr.w * cr.h;
+            L.ring = cd[c].ring.as<int32_t>();
+            L.slots = R;
+            L.w = cr.w;
+            L.h = cr.h;
+            L.sa = cd[c].sa.as<int32_t>();
+            L.sb = cd[c].sb.as<int32_t>();
+            L.psrc = lout.as<long long>() + cd[c].psrc;
+            L.ncentres = 0;
+            for (int i = 0; i < n; i++) {
+                const long f = f0 + i;
+                if (f < 2 * kRadius || !crop_active(c, f)) continue;
+                if (!L.ncentres) {
+                    L.centre0 = (int)(f - kRadius);
+                    L.slot0 = i;
+                }
+                L.ncentres++;
             }
+            L.nenc = 0;
+            for (int e = 0; e < E; e++) {
+                if (enc[e].crop != c) continue;
+                L.emid[L.nenc] = ed[e].mid.as<int32_t>();
+                L.penc[L.nenc] = lout.as<long long>() + ed[e].penc;
+                L.last_centre[L.nenc] = enc[e].d.length - 1 - kRadius;
+                L.nenc++;
+            }
+            (void)np;
         }
     }
 
@@ -638,6 +658,7 @@ private:
         case 0:
             hip_check(hipMemcpyAsync(desc.p, stage_buf[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
             launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
+            if (cfg.win > 16) launch_clear(lout.as<unsigned long long>(), (long long)lout_words, nullptr, 0, st);
             break;
         case 1: launch_cambi_base(pics, nullptr, P, max_base_tiles, st); break;
         case 2:
@@ -652,10 +673,7 @@ private:
             break;
         case 5: launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st); break;
         case 6: launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st); break;
-        case 7:
-            launch_persist(dptr<PersistJob>(o_psrc()), B * C, max_w, max_h, cfg.win, 1, st);
-            launch_persist(dptr<PersistJob>(o_penc()), B * E, max_w, max_h, cfg.win, 2, st);
-            break;
+        case 7: launch_lasting(dptr<LastingJob>(o_last()), C, max_w, max_h, cfg.win, st); break;
         case 8:
             launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 0, st);
             launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 1, st);

@@ -160,3 +160,20 @@ def test_profiled_graph_rounds_match_eager(gpu):
     assert np.array_equal(a0, a1) and np.array_equal(b0, b1) and np.array_equal(h0, h1)
     for k, v in s0.items():
         assert (v == s1[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(s1[k])), k
+
+
+@pytest.mark.parametrize("win", [32, 64])
+def test_batched_larger_windows_match_oracle(gpu, orc, win):
+    """32x32 and 64x64 windows (the lasting texture then adds patch sums atomically into cleared outputs)."""
+    n, w, h = 24, 320, 192
+    src, enc = frames_of(gpu, n, w=w, h=h, seed=31, full=(w, h))
+    run, cam, cam_src = oracle_run(orc, src, enc, 1440, win=win)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    sc = gpu.Scorer(gpu.ES_BATCHED, w, h, [gpu.EncodeDesc(w, h, n, w, 2, 0)], win=win, batch=8, use_graph=True)
+    sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
+    sc.sync()
+    got, hist = sc.segment(0, 0, with_hist=True)
+    ref = orc.accum_summary(run.segment(0))
+    compare_segment(got, ref)
+    assert np.array_equal(hist, ref[1])
