@@ -73,6 +73,26 @@ def config(n, es, frames):
     raise SystemExit(f"no configuration {n}")
 
 
+STAGE_KERNEL = {"cambi_cvalues": "cambi_cvalues_kernel", "cambi_base": "cambi_base_kernel",
+                "window_stats": "void window_stats_kernel<16, int, long long>", "persist": "void lasting_kernel<16>"}
+TRAFFIC_FILES = {(2, 3840, 2160, 16): "hbm_traffic_3840x2160_mosaic_batch16.json"}
+
+
+def measured_traffic(cfg, w, h, batch, stage):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
+    workload (scaled by the copy-kernel calibration recorded beside them), or (None, reason)."""
+    name = TRAFFIC_FILES.get((cfg, w, h, batch))
+    if name is None:
+        return None, "no FETCH_SIZE/WRITE_SIZE profile of this workload and batch"
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            rec = json.load(f)["kernels"][STAGE_KERNEL[stage]]
+    except (OSError, KeyError, ValueError):
+        return None, f"{name} holds no entry for {STAGE_KERNEL[stage]}"
+    return int(rec["bytes"]), f"profiles/{name}"
+
+
 def load_oracle():
     spec = importlib.util.spec_from_file_location("es_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
     mod = importlib.util.module_from_spec(spec)
@@ -93,6 +113,8 @@ def main():
     ap.add_argument("--pool", type=int, default=512, help="distinct frames generated per file at most")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run the calibration copy kernel over this many MiB (for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py runs on one GPU only: this path has no multi-GPU form here (asked for {args.gpus})", file=sys.stderr)
@@ -114,6 +136,13 @@ def main():
             frames.append(d)
         return frames
 
+    if args.copy_mib:
+        nbytes = args.copy_mib << 20
+        a, b = es.DeviceArray(nbytes, np.uint8), es.DeviceArray(nbytes, np.uint8)
+        for _ in range(3):
+            es.copy_kernel(b, a, nbytes)
+        es.synchronize()
+        del a, b
     src_frames = gen(src_d, sbytes)
     enc_frames = [gen(d, 2) for d, _ in encs]
     es.synchronize()
@@ -171,6 +200,7 @@ def main():
     dominant = max(alg_bytes, key=lambda k: stage[k])
     mean_ms = stage[dominant] / launches
     achieved = alg_bytes[dominant] / (mean_ms / 1e3) / 1e9
+    traffic, traffic_src = measured_traffic(args.config, W, H, args.batch, dominant)
     # the whole step's input bytes (each luma plane read once) against HBM
     b_in = sbytes * n_src + sum(2 * d.w * d.h for d, _ in encs)
     step_gbs = b_in * steps_per_s / 1e9
@@ -195,7 +225,8 @@ def main():
         "hbm_frac_b_in": round(step_gbs / HBM_PEAK_GBS, 6),
         "stage_ms_per_step": {k: round(v / args.steps, 5) for k, v in stage.items()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "mean_launch_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes[dominant])},
     }
 
