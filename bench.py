@@ -93,6 +93,17 @@ def measured_traffic(cfg, w, h, batch, stage):
     return int(rec["bytes"]), f"profiles/{name}"
 
 
+def host_cpu():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return f"{line.split(':', 1)[1].strip()}, {os.cpu_count()} logical CPUs visible"
+    except OSError:
+        pass
+    return None
+
+
 def load_oracle():
     spec = importlib.util.spec_from_file_location("es_oracle", os.path.join(ROOT, "oracle", "oracle.py"))
     mod = importlib.util.module_from_spec(spec)
@@ -190,11 +201,16 @@ def main():
     per_round_frames = args.batch
     pix = {"src": sum(cw * ch for cw, ch in crops), "enc": sum(d.w * d.h for d, _ in encs)}
     scale_px = lambda n: n * (1 + 1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
-    alg_bytes = {  # per launch (one round of `batch` frame steps)
-        "cambi_cvalues": 2 * scale_px(pix["src"] + pix["enc"]) * per_round_frames,
-        "cambi_base": (sbytes * pix["src"] + 2 * pix["enc"] + 2 * (pix["src"] + pix["enc"])) * per_round_frames,
-        "window_stats": (sbytes * pix["src"] + 2 * pix["enc"] + 4 * (len(crops) * 0 + pix["src"] + pix["enc"])) * per_round_frames,
-        "persist": (56 * pix["src"] + 8 * pix["enc"]) * per_round_frames,
+    B = per_round_frames
+    alg_bytes = {  # per launch (one round of B frame steps); each input read once, each output written once
+        # the packed planes of every scale of every picture
+        "cambi_cvalues": 2 * scale_px(pix["src"] + pix["enc"]) * B,
+        # luma in, packed scale-0 plane out
+        "cambi_base": (sbytes * pix["src"] + 2 * pix["enc"] + 2 * (pix["src"] + pix["enc"])) * B,
+        # luma of both pictures in, their int32 mid planes out
+        "window_stats": (sbytes * pix["src"] + 2 * pix["enc"] + 4 * (pix["src"] + pix["enc"])) * B,
+        # the B + 13 source mid planes the round's centres reach, SA and SB in and out, one encode mid plane per centre
+        "persist": 4 * ((B + 13 + 4) * pix["src"] + B * pix["enc"]),
     }
     launches = max(rounds, 1)
     dominant = max(alg_bytes, key=lambda k: stage[k])
@@ -271,11 +287,27 @@ def main():
         # pairs per second of the first encode, scaled to frame steps of this configuration's encodes
         pair_s = cpu_frames / cpu_t
         result["cpu_baseline"] = {"value": round(pair_s / len(encs), 4), "unit": "frame steps/s", "cores": threads,
-                                  "kind": "port",
+                                  "kind": "port", "host_cpu": host_cpu(),
                                   "sample": f"oracle run of the first {cpu_frames} frames of encode 0 ({d0.w}x{d0.h}), "
                                             f"{cpu_t:.1f} s; per-step rate divides by the {len(encs)} encode(s)"}
         gpu_cam, _ = bat.cambi(0, min(cpu_frames, descs[0].length))
         result["cambi_bit_exact_vs_oracle"] = bool(np.array_equal(gpu_cam, np.array(cam[: len(gpu_cam)])))
+        # the batched form's scores over the same sample against the oracle's (FP64 formulas on exact sums)
+        chk = es.Scorer(es.ES_BATCHED, W, H, [es.EncodeDesc(d0.w, d0.h, cpu_frames, d0.w, 2, 0)], batch=args.batch,
+                        use_graph=True, src_bytes=sbytes, src_shift=sshift)
+        s_, e_ = ptrs(0, cpu_frames)
+        chk.submit(s_, [row[:1] for row in e_])
+        chk.sync()
+        got, ref = chk.segment(0, -1), orc.accum_summary(run.merged())[0]
+
+        def err(a, b):
+            if np.isnan(a) and np.isnan(b):
+                return 0.0
+            return abs(a - b) if not (np.isnan(a) or np.isnan(b)) else float("inf")
+
+        result["score_err_vs_oracle"] = {k: err(got[k], ref[k]) for k in
+                                         ("cambi", "cambi_source", "texture", "banding", "grain", "score")}
+        result["score_err_vs_oracle"]["lasting_windows_equal"] = bool(got["n_lasting"] == ref["n_lasting"])
         print(f"[bench] cpu: {pair_s:.3f} pair-frames/s on {threads} threads", file=sys.stderr, flush=True)
 
     print(json.dumps(result))

@@ -161,6 +161,25 @@ def test_cvalue_of_a_two_level_step_is_the_closed_form(orc):
     assert cv[:, : 30 - pad - 1].max() == 0
 
 
+def test_reciprocal_table_is_correctly_rounded(orc):
+    """The reciprocal table (src/kernels.cu:5, :200) holds 1/i by design: every entry up to the 4K window's
+    (2 * 32 + 1)^2 + 1 = 4226 is the correctly rounded float32 quotient, checked against the exact rational 1/i.
+    The device computes the same quotient with a correctly rounded division."""
+    from fractions import Fraction
+    n = 4226
+    lut = orc.recip_lut(n)
+    assert lut[0] == 0
+    for i in range(1, n):
+        v = float(lut[i])
+        exact = Fraction(1, i)
+        # no float32 lies strictly closer to 1/i than v: its neighbours are at least as far
+        lo = float(np.nextafter(np.float32(v), np.float32(0)))
+        hi = float(np.nextafter(np.float32(v), np.float32(2)))
+        err = abs(Fraction(v) - exact)
+        assert err <= abs(Fraction(lo) - exact) and err <= abs(Fraction(hi) - exact), i
+    assert np.array_equal(lut[1:], np.float32(1) / np.arange(1, n, dtype=np.float32))
+
+
 def test_flat_picture_has_full_mask_and_no_banding(orc):
     img = np.full((72, 128), 500, np.uint16)
     p = orc.params(128, 72)
