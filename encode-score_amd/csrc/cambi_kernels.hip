@@ -173,30 +173,59 @@ void launch_cambi_scale(const CambiPic *pics, const long long * /*px_first*/, in
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int kCvMaxB = 32;
 constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
-constexpr int kCvCols = 130;    // prefix entries per number pair: columns 0 .. 128, padded
+constexpr int kCvCols = 130;    // prefix entries per number: columns 0 .. 128, padded to an even count
 
 struct CvShared {
-    uint32_t cc[kCvMaxB / 4][128];            // column counts: numbers 4q .. 4q + 3 of column c in cc[q][c]
-    uint32_t pp[kCvMaxB / 2][kCvCols];        // prefix over columns: numbers 2j, 2j + 1 in pp[j][c]
+    uint32_t cc[kCvMaxB / 4][128];            // column counts: numbers 4q .. 4q + 3 of column c in cc[q][c], u8 each
+    uint16_t pp[kCvMaxB][kCvCols];            // prefix over columns of the column counts of number b: pp[b][c]
     uint16_t rank[kMaxBand + 2 * kDiffs];     // number of value v at rank[v + 4], kNone if not relevant
 };
-static_assert(sizeof(uint32_t) * (kCvMaxB / 2) * kCvCols >= 2 * kCvFlags, "flags live in pp during the setup");
+static_assert(sizeof(uint16_t) * kCvMaxB * kCvCols >= 2 * kCvFlags, "flags live in pp during the setup");
+// Column counts are at most 2 pad + 1 <= 65 (u8), and a prefix over 128 columns at most 128 * 65 < 2^16 (u16).
+static_assert(2 * kMaxPad + 1 < 256 && 128 * (2 * kMaxPad + 1) < 65536, "count widths");
 
 int cvalue_tiles(int w, int h, int tile_h) { return ((w + kCvTileW - 1) / kCvTileW) * ((h + tile_h - 1) / tile_h); }
 
+// Global-memory pointers: loads and stores through them are global_* instructions, which (unlike flat ones) do not
+// hold up the waits on LDS results.
+#define ES_GLOBAL __attribute__((address_space(1)))
+
 struct CvTile {
-    const uint16_t *img;
+    const ES_GLOBAL uint16_t *img;
+    const ES_GLOBAL uint16_t *zero;  // a 0 sample (inactive) in device memory
     int w, h, x0, y0, rows;
     int pad, band_base, band_size, vlt;
     int tvi[kDiffs], weight[kDiffs];
-    float *plane;  // full c-value plane or nullptr
-    float *seg;    // this tile's segment of the list
+    ES_GLOBAL float *plane;  // full c-value plane or nullptr
+    ES_GLOBAL float *seg;    // this tile's segment of the list
 };
 
-// The raw packed sample at (x, y), or 0 (mask bit clear: never active) outside the picture.
+// The raw packed sample at (x, y), or 0 (mask bit clear: never active) outside the picture.  Every lane loads, from
+// a zero sample when (x, y) is outside: no lane mask is set up around the load and no select waits for its result.
 __device__ __forceinline__ unsigned cv_raw(const CvTile &t, int x, int y)
 {
-    return (x >= 0 && x < t.w && y >= 0 && y < t.h) ? (unsigned)t.img[(size_t)y * t.w + x] : 0u;
+    const bool in = (unsigned)x < (unsigned)t.w && (unsigned)y < (unsigned)t.h;
+    return *(in ? t.img + (unsigned)(y * t.w + x) : t.zero);
+}
+
+// Index into the rank table of a packed sample: value - band_base + kDiffs when it is active, else 0 (whose entry is
+// kNone: no value below the band is relevant).
+__device__ __forceinline__ unsigned cv_index(const CvTile &t, unsigned v)
+{
+    const unsigned u = (v & 0x7FFFu) - (unsigned)t.band_base;
+    return (v >> 15) && u < (unsigned)t.band_size ? u + kDiffs : 0u;
+}
+
+// RN(1/n) for 1 <= n <= 8192, the entry of the reference's reciprocal table: the hardware reciprocal (within 1 ulp)
+// refined by one Newton step with fused multiply-adds.  The residual e = 1 - n r0 is exact, and r0 + e r0 differs from
+// 1/n by r0 e^2 / (1 - e), below 2^-44 relative, while 1/n lies at least 2^-37 relative from any rounding boundary
+// (n < 2^13), so the rounded step is 1/n correctly rounded.
+__device__ __forceinline__ float recip_rn(unsigned n)
+{
+    const float x = (float)n;
+    const float r0 = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r0, 1.0f);
+    return __builtin_fmaf(e, r0, r0);
 }
 
 // The value less band_base of an active (masked, in-band) packed sample, else kNone.
@@ -209,19 +238,21 @@ __device__ __forceinline__ unsigned cv_decode(const CvTile &t, unsigned v)
 template <int B>
 __device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
 {
-    constexpr int Q = B / 4, W2 = B / 2;
+    constexpr int Q = B / 4, NC = B / 8;  // count words per column; chunks of 8 numbers (2 words, 4 pairs)
     const int lane = threadIdx.x, pad = t.pad, span = 2 * pad + 1;
     const int c0 = 2 * lane, xl = t.x0 - pad + c0;  // this lane's margin columns c0, c0 + 1 at picture column xl
     const bool col0 = c0 < kCvTileW + 2 * pad, col1 = c0 + 1 < kCvTileW + 2 * pad;
+    const int x = t.x0 + lane;  // this lane's output column
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
 #pragma unroll
         for (int q = 0; q < Q; q++) *(uint2 *)&sh.cc[q][c0] = make_uint2(0, 0);
-        auto add = [&](unsigned raw, int c, unsigned delta) {
-            const unsigned u = cv_decode(t, raw);
-            if (u == kNone) return;
-            const unsigned b = (unsigned)sh.rank[u + kDiffs] - (unsigned)b0;
-            if (b < (unsigned)B) atomicAdd(&sh.cc[b >> 2][c], delta << (8 * (b & 3)));
+        // A sample of rank rk entering (delta 1) or leaving (delta ~0) column c's count: an increment of zero when
+        // it is inactive or outside the pass, so that every lane makes the same LDS operation.
+        auto bump = [&](unsigned rk, int c, unsigned delta) {
+            const unsigned b = rk - (unsigned)b0;
+            const unsigned inc = b < (unsigned)B ? delta << ((b & 3u) << 3) : 0u;
+            atomicAdd(&sh.cc[(b >> 2) & (Q - 1)][c], inc);
         };
         // the window rows above the first row, eight at a time
         const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
@@ -229,106 +260,139 @@ __device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned 
             unsigned v[16];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                v[2 * j] = y + j < yb && col0 ? cv_raw(t, xl, y + j) : 0u;
-                v[2 * j + 1] = y + j < yb && col1 ? cv_raw(t, xl + 1, y + j) : 0u;
+                const int yy = y + j < yb ? y + j : -1;
+                v[2 * j] = cv_raw(t, col0 ? xl : -1, yy);
+                v[2 * j + 1] = cv_raw(t, col1 ? xl + 1 : -1, yy);
             }
+            unsigned rk[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) rk[j] = sh.rank[cv_index(t, v[j])];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                add(v[2 * j], c0, 1u);
-                add(v[2 * j + 1], c0 + 1, 1u);
+                bump(rk[2 * j], c0, 1u);
+                bump(rk[2 * j + 1], c0 + 1, 1u);
             }
         }
-        // codes of the row: entering (y + pad), own (y), leaving (y - pad); loaded a row ahead
+        // codes of the row: entering (y + pad), own (y), leaving (y - pad); loaded a row ahead.  The leaving samples
+        // of row y are removed at the start of row y + 1, with that row's entering ones.
         auto load = [&](int y, unsigned *v) {
-            v[0] = col0 ? cv_raw(t, xl, y + pad) : 0u;
-            v[1] = col1 ? cv_raw(t, xl + 1, y + pad) : 0u;
-            v[2] = cv_raw(t, t.x0 + lane, y);
-            v[3] = col0 ? cv_raw(t, xl, y - pad) : 0u;
-            v[4] = col1 ? cv_raw(t, xl + 1, y - pad) : 0u;
+            v[0] = cv_raw(t, col0 ? xl : -1, y + pad);
+            v[1] = cv_raw(t, col1 ? xl + 1 : -1, y + pad);
+            v[2] = cv_raw(t, x, y);
+            v[3] = cv_raw(t, col0 ? xl : -1, y - pad);
+            v[4] = cv_raw(t, col1 ? xl + 1 : -1, y - pad);
         };
-        unsigned cur[5], nxt[5];
+        unsigned cur[5], nxt[5] = {0, 0, 0, 0, 0}, gone0 = 0, gone1 = 0;
         load(t.y0, cur);
         for (int r = 0; r < t.rows; r++) {
             const int y = t.y0 + r;
             if (r + 1 < t.rows) load(y + 1, nxt);
-            add(cur[0], c0, 1u);
-            add(cur[1], c0 + 1, 1u);
-            const unsigned u = cv_decode(t, cur[2]);
-            const unsigned own = u == kNone ? (unsigned)kNone : sh.rank[u + kDiffs];
+            // every rank the row needs, read at once: the samples entering and leaving, the own sample and the
+            // values within kDiffs of it
+            const unsigned io = cv_index(t, cur[2]), base = io ? io - kDiffs : 0u;
+            const unsigned re0 = sh.rank[cv_index(t, cur[0])], re1 = sh.rank[cv_index(t, cur[1])];
+            const unsigned rl0 = sh.rank[cv_index(t, gone0)], rl1 = sh.rank[cv_index(t, gone1)];
+            unsigned rk[2 * kDiffs + 1];
+#pragma unroll
+            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = sh.rank[base + k];
+            bump(rl0, c0, ~0u);
+            bump(rl1, c0 + 1, ~0u);
+            bump(re0, c0, 1u);
+            bump(re1, c0 + 1, 1u);
+            const unsigned own = io ? rk[kDiffs] : (unsigned)kNone;
             const bool serve = own != kNone && (int)own >= lo && (int)own < hi;
-            if (__any(serve)) {
-                // column counts of both columns, all number pairs at once, widened to 16 bits per number
-                uint32_t x[W2], e[W2];
+            if (__builtin_amdgcn_ballot_w64(serve)) {
+                // the chunks of 8 numbers some served pixel reads: its own number and those within 4 of it
+                const int rel = (int)own - b0;
+                unsigned need = 1;
+                if (NC > 1) {
+                    need = 0;
 #pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const uint2 cq = *(const uint2 *)&sh.cc[q][c0];
+                    for (int k = 0; k < NC; k++)
+                        need |= __builtin_amdgcn_ballot_w64(serve && rel + kDiffs >= 8 * k && rel - kDiffs < 8 * k + 8) ? 1u << k : 0u;
+                }
+                uint2 cw[Q];
 #pragma unroll
-                    for (int hf = 0; hf < 2; hf++) {
-                        const uint32_t ra = cq.x >> (16 * hf), rb = cq.y >> (16 * hf);
-                        e[2 * q + hf] = (ra & 0xFFu) | ((ra & 0xFF00u) << 8);              // column c0
-                        x[2 * q + hf] = e[2 * q + hf] + ((rb & 0xFFu) | ((rb & 0xFF00u) << 8));  // both columns
+                for (int q = 0; q < Q; q++) cw[q] = *(const uint2 *)&sh.cc[q][c0];
+#pragma unroll
+                for (int k = 0; k < NC; k++) {
+                    if (!(need >> k & 1u)) continue;
+                    // column counts of both columns, numbers 8k .. 8k + 7 widened to 16 bits per number: pair j
+                    // holds numbers 8k + 2j (low half) and 8k + 2j + 1 (high half)
+                    const uint2 ca = cw[2 * k], cb = cw[2 * k + 1];
+                    uint32_t xs[4], e[4], s[4];
+                    e[0] = __builtin_amdgcn_perm(0u, ca.x, 0x0C010C00u);
+                    e[1] = __builtin_amdgcn_perm(0u, ca.x, 0x0C030C02u);
+                    e[2] = __builtin_amdgcn_perm(0u, cb.x, 0x0C010C00u);
+                    e[3] = __builtin_amdgcn_perm(0u, cb.x, 0x0C030C02u);
+                    xs[0] = e[0] + __builtin_amdgcn_perm(0u, ca.y, 0x0C010C00u);
+                    xs[1] = e[1] + __builtin_amdgcn_perm(0u, ca.y, 0x0C030C02u);
+                    xs[2] = e[2] + __builtin_amdgcn_perm(0u, cb.y, 0x0C010C00u);
+                    xs[3] = e[3] + __builtin_amdgcn_perm(0u, cb.y, 0x0C030C02u);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) s[j] = xs[j];
+#define ES_SCAN_STEP(ctrl, rmask)                                                                        \
+    _Pragma("unroll") for (int j = 0; j < 4; j++) s[j] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s[j], ctrl, rmask, 0xF, false);
+                    ES_SCAN_STEP(0x111, 0xF)
+                    ES_SCAN_STEP(0x112, 0xF)
+                    ES_SCAN_STEP(0x114, 0xF)
+                    ES_SCAN_STEP(0x118, 0xF)
+                    ES_SCAN_STEP(0x142, 0xA)
+                    ES_SCAN_STEP(0x143, 0xC)
+#undef ES_SCAN_STEP
+                    // P(c0) = s - x, P(c0 + 1) = P(c0) + count(c0), P(c0 + 2) = s (the next lane's P(c0) as well,
+                    // written by both with the same value), split into one u16 row per number
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const uint32_t p = s[j] - xs[j], q = p + e[j];
+                        const int n = 8 * k + 2 * j;
+                        *(uint32_t *)&sh.pp[n][c0] = __builtin_amdgcn_perm(q, p, 0x05040100u);
+                        *(uint32_t *)&sh.pp[n + 1][c0] = __builtin_amdgcn_perm(q, p, 0x07060302u);
+                        sh.pp[n][c0 + 2] = (uint16_t)s[j];
+                        sh.pp[n + 1][c0 + 2] = (uint16_t)(s[j] >> 16);
                     }
                 }
-                uint32_t s[W2];
-#pragma unroll
-                for (int j = 0; j < W2; j++) s[j] = x[j];
-#define ES_SCAN_STEP(ctrl, rmask)                                                                        \
-    _Pragma("unroll") for (int j = 0; j < W2; j++) s[j] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)s[j], ctrl, rmask, 0xF, false);
-                ES_SCAN_STEP(0x111, 0xF)
-                ES_SCAN_STEP(0x112, 0xF)
-                ES_SCAN_STEP(0x114, 0xF)
-                ES_SCAN_STEP(0x118, 0xF)
-                ES_SCAN_STEP(0x142, 0xA)
-                ES_SCAN_STEP(0x143, 0xC)
-#undef ES_SCAN_STEP
-                // P(c0) = s - x, P(c0 + 1) = P(c0) + count(c0), P(c0 + 2) = s (the next lane's P(c0) as well)
-#pragma unroll
-                for (int j = 0; j < W2; j++) {
-                    const uint32_t p = s[j] - x[j];
-                    *(uint2 *)&sh.pp[j][c0] = make_uint2(p, p + e[j]);
-                }
-#pragma unroll
-                for (int j = 0; j < W2; j++) sh.pp[j][c0 + 2] = s[j];
                 __syncthreads();  // the prefixes of all columns are in before any lane reads them
-                // the numbers of p0 and of the neighbours at +-(d + 1), and their window counts
-                const unsigned uu = serve ? u : 0u;
-                unsigned rk[2 * kDiffs + 1];
-                rk[kDiffs] = own;
-#pragma unroll
-                for (int d = 0; d < kDiffs; d++) {
-                    rk[kDiffs + d + 1] = sh.rank[uu + kDiffs + d + 1];
-                    rk[kDiffs - d - 1] = sh.rank[uu + kDiffs - d - 1];
-                }
+                // window counts of p0 and of the neighbours at +-(d + 1): P(x + span) - P(x) in margin columns; a
+                // number outside the pass reads a valid entry and counts 0
+                const uint16_t *p1 = &sh.pp[0][0] + lane, *p2 = p1 + span;
                 unsigned cnt[2 * kDiffs + 1];
 #pragma unroll
                 for (int k = 0; k < 2 * kDiffs + 1; k++) {
-                    const unsigned b = rk[k] - (unsigned)b0, bb = b < (unsigned)B ? b : 0u;
-                    const uint32_t dlt = sh.pp[bb >> 1][lane + span] - sh.pp[bb >> 1][lane];
-                    cnt[k] = b < (unsigned)B ? (dlt >> (16 * (bb & 1))) & 0xFFFFu : 0u;
+                    const unsigned b = rk[k] - (unsigned)b0, bb = b & (B - 1);
+                    const unsigned d = (unsigned)p2[bb * kCvCols] - (unsigned)p1[bb * kCvCols];
+                    cnt[k] = b < (unsigned)B ? d : 0u;
                 }
-                const int value = (int)uu + t.band_base + kDiffs;
+                // every count is needed below on some lanes: have all of them read here, together, rather than each
+                // under its own condition
+#pragma unroll
+                for (int k = 0; k < 2 * kDiffs + 1; k++) asm volatile("" ::"v"(cnt[k]));
+                const int value = (int)(io ? io - kDiffs : 0u) + t.band_base + kDiffs;
                 const unsigned p0 = cnt[kDiffs];
                 float c = 0.0f;
 #pragma unroll
                 for (int d = 0; d < kDiffs; d++) {
-                    const unsigned p1 = cnt[kDiffs + d + 1], p2 = cnt[kDiffs - d - 1];
-                    const unsigned pm = p1 > p2 ? p1 : p2;
-                    // w * p0 * pm * RN(1 / (p0 + pm)): the reciprocal correctly rounded, as libvmaf's table holds it
-                    const float rcp = __fdiv_rn(1.0f, (float)(p0 + pm));
-                    const float val = __fmul_rn((float)((unsigned)t.weight[d] * p0 * pm), rcp);
+                    const unsigned pa = cnt[kDiffs + d + 1], pb = cnt[kDiffs - d - 1];
+                    const unsigned pm = pa > pb ? pa : pb;
+                    // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; p0 + pm is
+                    // at least 1 on a served pixel (its own count), and other lanes' results are discarded
+                    const float rcp = recip_rn(p0 + pm);
+                    const float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
                     const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt && pm > 0;
                     c = on && val > c ? val : c;
                 }
-                if (!serve) c = 0.0f;
-                if (serve && t.plane) t.plane[(size_t)y * t.w + t.x0 + lane] = c;
-                if (c > 0.0f) sum_fx += fixed24(c);
-                const unsigned long long m = __ballot(c > 0.0f);
+                c = serve ? c : 0.0f;
+                if (t.plane) {
+                    if (serve) t.plane[(unsigned)(y * t.w + x)] = c;
+                }
+                sum_fx += c > 0.0f ? fixed24(c) : 0ull;
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
                 if (c > 0.0f) t.seg[tcount + __popcll(m & ((1ull << lane) - 1))] = c;
                 tcount += (unsigned)__popcll(m);
                 __syncthreads();  // every lane has read the prefixes before the next row writes them
             }
-            add(cur[3], c0, ~0u);
-            add(cur[4], c0 + 1, ~0u);
+            gone0 = cur[3];
+            gone1 = cur[4];
 #pragma unroll
             for (int k = 0; k < 5; k++) cur[k] = nxt[k];
         }
@@ -354,7 +418,8 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
     CvTile t;
     t.w = pic.sw[s];
     t.h = pic.sh[s];
-    t.img = pic.planes + pic.off[s];
+    t.img = (const ES_GLOBAL uint16_t *)(pic.planes + pic.off[s]);
+    t.zero = (const ES_GLOBAL uint16_t *)K.zero;
     const int across = (t.w + kCvTileW - 1) / kCvTileW;
     t.x0 = (local % across) * kCvTileW;
     t.y0 = (local / across) * tile_h;
@@ -368,8 +433,8 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
         t.tvi[d] = K.tvi[d];
         t.weight[d] = K.weight[d];
     }
-    t.plane = pic.cvals ? pic.cvals + pic.off[s] : nullptr;
-    t.seg = pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h;
+    t.plane = pic.cvals ? (ES_GLOBAL float *)(pic.cvals + pic.off[s]) : nullptr;
+    t.seg = (ES_GLOBAL float *)(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h);
     unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
     const int lane = threadIdx.x, nv = t.band_size + 2 * kDiffs;
     const int c0 = 2 * lane, xl = t.x0 - t.pad + c0, cw = kCvTileW + 2 * t.pad;

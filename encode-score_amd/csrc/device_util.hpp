@@ -31,8 +31,13 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
-// A positive c-value (at least 0.5) as an exact integer in units of 2^-24.
-__device__ __forceinline__ unsigned long long fixed24(float c) { return (unsigned long long)((double)c * 16777216.0); }
+// A positive c-value (at least 0.5) as an exact integer in units of 2^-24: with c = m 2^(e - 150), m the 24-bit
+// significand and e >= 126 the biased exponent, c 2^24 = m << (e - 126).
+__device__ __forceinline__ unsigned long long fixed24(float c)
+{
+    const unsigned b = __float_as_uint(c);
+    return (unsigned long long)((b & 0x7FFFFFu) | 0x800000u) << ((b >> 23) - 126u);
+}
 
 // Histogram::add's bin (src/stats.hpp:16): 0.05-point bins over [-500, 500], rounded half away from zero.
 __device__ __forceinline__ int hist_bin(double pct)
