@@ -191,8 +191,7 @@ int cvalue_tiles(int w, int h, int tile_h) { return ((w + kCvTileW - 1) / kCvTil
 #define ES_GLOBAL __attribute__((address_space(1)))
 
 struct CvTile {
-    const ES_GLOBAL uint16_t *img;
-    const ES_GLOBAL uint16_t *zero;  // a 0 sample (inactive) in device memory
+    __amdgpu_buffer_rsrc_t rs;  // the scale's packed plane as a buffer of w * h samples
     int w, h, x0, y0, rows;
     int pad, band_base, band_size, vlt;
     int tvi[kDiffs], weight[kDiffs];
@@ -200,12 +199,15 @@ struct CvTile {
     ES_GLOBAL float *seg;    // this tile's segment of the list
 };
 
-// The raw packed sample at (x, y), or 0 (mask bit clear: never active) outside the picture.  Every lane loads, from
-// a zero sample when (x, y) is outside: no lane mask is set up around the load and no select waits for its result.
-__device__ __forceinline__ unsigned cv_raw(const CvTile &t, int x, int y)
+// Samples are read as buffer loads at byte offset xo + yo: a load past the buffer's w * h samples returns 0, which is
+// an inactive sample (mask bit clear), so positions outside the picture need no test and no lane mask.  A column
+// outside the picture gets xo = 2^31, and a row outside it a yo of at least 2 w h (below) or of 2^32 - 2 w |y| with
+// |y| <= 2 kMaxPad + 1 (above); every sum with such a term lies at or past 2 w h for pictures under 2^30 bytes.
+constexpr unsigned kCvOut = 0x80000000u;
+__device__ __forceinline__ unsigned cv_col(const CvTile &t, int x) { return (unsigned)x < (unsigned)t.w ? 2u * (unsigned)x : kCvOut; }
+__device__ __forceinline__ unsigned cv_at(const CvTile &t, unsigned xo, int y)
 {
-    const bool in = (unsigned)x < (unsigned)t.w && (unsigned)y < (unsigned)t.h;
-    return *(in ? t.img + (unsigned)(y * t.w + x) : t.zero);
+    return __builtin_amdgcn_raw_buffer_load_b16(t.rs, xo + (unsigned)(y * t.w * 2), 0, 0);
 }
 
 // Index into the rank table of a packed sample: value - band_base + kDiffs when it is active, else 0 (whose entry is
@@ -243,6 +245,8 @@ __device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned 
     const int c0 = 2 * lane, xl = t.x0 - pad + c0;  // this lane's margin columns c0, c0 + 1 at picture column xl
     const bool col0 = c0 < kCvTileW + 2 * pad, col1 = c0 + 1 < kCvTileW + 2 * pad;
     const int x = t.x0 + lane;  // this lane's output column
+    // byte offsets of this lane's margin columns and output column (kCvOut: none, or outside the picture)
+    const unsigned xo0 = col0 ? cv_col(t, xl) : kCvOut, xo1 = col1 ? cv_col(t, xl + 1) : kCvOut, xoo = cv_col(t, x);
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
 #pragma unroll
@@ -261,8 +265,8 @@ __device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned 
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int yy = y + j < yb ? y + j : -1;
-                v[2 * j] = cv_raw(t, col0 ? xl : -1, yy);
-                v[2 * j + 1] = cv_raw(t, col1 ? xl + 1 : -1, yy);
+                v[2 * j] = cv_at(t, xo0, yy);
+                v[2 * j + 1] = cv_at(t, xo1, yy);
             }
             unsigned rk[16];
 #pragma unroll
@@ -276,11 +280,11 @@ __device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned 
         // codes of the row: entering (y + pad), own (y), leaving (y - pad); loaded a row ahead.  The leaving samples
         // of row y are removed at the start of row y + 1, with that row's entering ones.
         auto load = [&](int y, unsigned *v) {
-            v[0] = cv_raw(t, col0 ? xl : -1, y + pad);
-            v[1] = cv_raw(t, col1 ? xl + 1 : -1, y + pad);
-            v[2] = cv_raw(t, x, y);
-            v[3] = cv_raw(t, col0 ? xl : -1, y - pad);
-            v[4] = cv_raw(t, col1 ? xl + 1 : -1, y - pad);
+            v[0] = cv_at(t, xo0, y + pad);
+            v[1] = cv_at(t, xo1, y + pad);
+            v[2] = cv_at(t, xoo, y);
+            v[3] = cv_at(t, xo0, y - pad);
+            v[4] = cv_at(t, xo1, y - pad);
         };
         unsigned cur[5], nxt[5] = {0, 0, 0, 0, 0}, gone0 = 0, gone1 = 0;
         load(t.y0, cur);
@@ -418,8 +422,14 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
     CvTile t;
     t.w = pic.sw[s];
     t.h = pic.sh[s];
-    t.img = (const ES_GLOBAL uint16_t *)(pic.planes + pic.off[s]);
-    t.zero = (const ES_GLOBAL uint16_t *)K.zero;
+    {
+        // one tile per wave: the plane's address is the same on every lane, which keeps the descriptor in scalar
+        // registers
+        const unsigned long long a = (unsigned long long)(pic.planes + pic.off[s]);
+        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+        t.rs = __builtin_amdgcn_make_buffer_rsrc((void *)((unsigned long long)hi << 32 | lo), (short)0,
+                                                 __builtin_amdgcn_readfirstlane(t.w * t.h * 2), 0x00020000);
+    }
     const int across = (t.w + kCvTileW - 1) / kCvTileW;
     t.x0 = (local % across) * kCvTileW;
     t.y0 = (local / across) * tile_h;
@@ -446,7 +456,7 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
     for (int r = 0; r < t.rows; r += 8) {
         unsigned v[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = r + j < t.rows ? cv_raw(t, t.x0 + lane, t.y0 + r + j) : 0u;
+        for (int j = 0; j < 8; j++) v[j] = cv_at(t, cv_col(t, t.x0 + lane), r + j < t.rows ? t.y0 + r + j : -1);
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const unsigned u = cv_decode(t, v[j]);
@@ -465,8 +475,8 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
         unsigned v[16];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
-            v[2 * j] = y + j < yb && c0 < cw ? cv_raw(t, xl, y + j) : 0u;
-            v[2 * j + 1] = y + j < yb && c0 + 1 < cw ? cv_raw(t, xl + 1, y + j) : 0u;
+            v[2 * j] = cv_at(t, c0 < cw ? cv_col(t, xl) : kCvOut, y + j < yb ? y + j : -1);
+            v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
         }
 #pragma unroll
         for (int j = 0; j < 16; j++) {

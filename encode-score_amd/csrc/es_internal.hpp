@@ -34,7 +34,6 @@ inline void hip_check(hipError_t r, const char *what)
 struct CambiConsts {
     int pad, band_base, band_size, vlt, mask_index, window;
     int tvi[kDiffs], weight[kDiffs];
-    uint16_t zero[4];  // always 0: the sample the c-value kernel reads for positions outside the picture
 };
 
 // A luma plane as the kernels read it: sample(x, y) = plane[y * pitch + x] << shift, `bytes` 1 or 2 per sample, and
