@@ -186,10 +186,6 @@ static_assert(2 * kMaxPad + 1 < 256 && 128 * (2 * kMaxPad + 1) < 65536, "count w
 
 int cvalue_tiles(int w, int h, int tile_h) { return ((w + kCvTileW - 1) / kCvTileW) * ((h + tile_h - 1) / tile_h); }
 
-// Global-memory pointers: loads and stores through them are global_* instructions, which (unlike flat ones) do not
-// hold up the waits on LDS results.
-#define ES_GLOBAL __attribute__((address_space(1)))
-
 struct CvTile {
     __amdgpu_buffer_rsrc_t rs;  // the scale's packed plane as a buffer of w * h samples
     int w, h, x0, y0, rows;

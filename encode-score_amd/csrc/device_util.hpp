@@ -2,12 +2,16 @@
 #pragma once
 #include "es_internal.hpp"
 
+// Global-memory pointers: loads and stores through them are global_* instructions, which (unlike flat ones) do not
+// hold up the waits on LDS results.
+#define ES_GLOBAL __attribute__((address_space(1)))
+
 namespace es {
 
 __device__ __forceinline__ unsigned raw_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = in.bytes == 1 ? ((const uint8_t *)in.ptr)[i] : ((const uint16_t *)in.ptr)[i];
+    const unsigned v = in.bytes == 1 ? ((const ES_GLOBAL uint8_t *)in.ptr)[i] : ((const ES_GLOBAL uint16_t *)in.ptr)[i];
     return v << in.shift;
 }
 

@@ -39,7 +39,7 @@ __device__ __forceinline__ void push(T (&a)[N], T v)
 __device__ __forceinline__ int in_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = in.bytes == 1 ? ((const uint8_t *)in.ptr)[i] : ((const uint16_t *)in.ptr)[i];
+    const unsigned v = in.bytes == 1 ? ((const ES_GLOBAL uint8_t *)in.ptr)[i] : ((const ES_GLOBAL uint16_t *)in.ptr)[i];
     return (int)(v << in.shift);
 }
 
@@ -154,13 +154,162 @@ __global__ __launch_bounds__(256) void window_stats_kernel(const WsPair *pairs)
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// Integer form.  One wave per strip of 64 columns and kWsStrip rows, on its own (64-thread blocks, no barrier between
+// waves).  Row r is staged through LDS with 4 columns of margin on either side; the next row's samples are loaded
+// while the current one is processed.  Per picture a lane keeps, in registers, the horizontal 3-sums (centred on its
+// column and on either neighbour), 9-sums and samples of the last rows in rings whose slots are fixed by r mod 9 (the
+// row loop is unrolled 9 times, so the rings rotate by renaming).  The sums are taken as soon as their rows are in:
+// F = 9 y - S3 of row r - 1, GX / GY of row r - 2 (GY needs S3 of the row below), M = 9 S3 - S9 of row r - 4.  No
+// product mixes those groups, so each group keeps its own 16-row partial sums (32-bit, exact: |F|, |GX|, |GY| <=
+// 9 * 1023, 16 of their products < 2^31; M's products are 64-bit) and flushes them when its row closes a block.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int kWsStrip = 64;
+
+
+template <int WIN>
+__global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pairs)
+{
+    static_assert(kWsStrip % WIN == 0 && WIN % 16 == 0, "windows tile the strip in 16-row blocks");
+    __shared__ int rowbuf[2][2][72 + 64];  // [buffer][picture][column - (x0 - 4)]; 72 .. 135 take unused halo stores
+    const WsPair P = pairs[blockIdx.z];  // a copy: the stores below cannot change it, so it is read once
+    const int lane = threadIdx.x;
+    const int w = P.w, h = P.h, ww = w / WIN, H = h / WIN * WIN, W = ww * WIN;
+    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kWsStrip;
+    if (x0 >= W || y0 >= H) return;  // uniform over the block
+    const int y1 = min(y0 + kWsStrip, H), rend = y1 + 3;  // rows r = y0 - 4 .. rend
+    const int x = x0 + lane;
+    const bool inside = x < W;
+    // this lane's sample column and, for lanes 0 .. 7, a margin column (x0 - 4 .. x0 - 1, x0 + 64 .. x0 + 67)
+    const int xa = min(x, w - 1);
+    const int xh = lane < 4 ? max(x0 - 4 + lane, 0) : min(x0 + 60 + lane, w - 1);
+    const int hslot = lane < 4 ? lane : lane < 8 ? lane + 64 : 72 + lane;
+    const PlaneIn in[2] = {P.r, P.d};
+    int32_t *mid[2] = {P.mid_r, P.mid_d};
+    auto sample = [&](int p, int xx, int r) { return in_sample(in[p], xx, min(max(r, 0), h - 1)) << P.up; };
+    int pr[2], ph[2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        pr[p] = sample(p, xa, y0 - 4);
+        ph[p] = sample(p, xh, y0 - 4);
+    }
+    int h3c[2][9], h3l[2][3], h3r[2][3], h9[2][9], yv[2][3], s9[2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+#pragma unroll
+        for (int k = 0; k < 9; k++) h3c[p][k] = h9[p][k] = 0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) h3l[p][k] = h3r[p][k] = yv[p][k] = 0;
+        s9[p] = 0;
+    }
+    int accF[3] = {0, 0, 0}, accG[6] = {0, 0, 0, 0, 0, 0};
+    long long accM[3] = {0, 0, 0};
+    long long winF[3] = {0, 0, 0}, winG[6] = {0, 0, 0, 0, 0, 0}, winM[3] = {0, 0, 0};  // WIN > 16 only
+    // a closed 16-row block of one group: straight to its windows (WIN = 16) or into the lane's window sums
+    auto flush = [&](auto &acc, auto &win, const auto &slot, int y) {
+        constexpr int N = sizeof(acc) / sizeof(acc[0]);
+#pragma unroll
+        for (int k = 0; k < N; k++) {
+            win[k] += (long long)acc[k];
+            acc[k] = 0;
+        }
+        if ((y + 1) % WIN) return;  // uniform
+#pragma unroll
+        for (int k = 0; k < N; k++) win[k] = reduce_lanes(win[k], WIN);
+        if (inside && lane % WIN == 0) {
+            ES_GLOBAL long long *o = (ES_GLOBAL long long *)P.sums + ((size_t)(y / WIN) * ww + x / WIN) * 12;
+#pragma unroll
+            for (int k = 0; k < N; k++) o[slot[k]] = win[k];
+        }
+#pragma unroll
+        for (int k = 0; k < N; k++) win[k] = 0;
+    };
+    const int slotF[3] = {0, 2, 4}, slotG[6] = {6, 7, 8, 9, 10, 11}, slotM[3] = {1, 3, 5};
+    int buf = 0;
+    for (int base = y0 - 4; base <= rend; base += 9) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            // whole groups of 9 rows: rows past rend read clamped samples and produce nothing
+            const int r = base + i;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                rowbuf[buf][p][lane + 4] = pr[p];
+                rowbuf[buf][p][hslot] = ph[p];
+            }
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                pr[p] = sample(p, xa, r + 1);
+                ph[p] = sample(p, xh, r + 1);
+            }
+            __syncthreads();  // the row is in (one wave per block: no other wave waits here)
+            int F[2], GX[2], GY[2], M[2];
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                int v[9];
+#pragma unroll
+                for (int d = 0; d < 9; d++) v[d] = rowbuf[buf][p][lane + d];  // columns x - 4 .. x + 4
+                const int a = v[3] + v[4];
+                const int hc = a + v[5], hl = a + v[2], hr = v[4] + v[5] + v[6];
+                const int h9v = hl + (v[5] + v[6]) + (v[0] + v[1]) + (v[7] + v[8]);
+                // slots: row r - k sits in h3c / h9 slot (i - k) mod 9 and in the 3-slot rings at (i - k) mod 3
+                const int c1 = h3c[p][(i + 8) % 9], c2 = h3c[p][(i + 7) % 9], c3 = h3c[p][(i + 6) % 9];
+                const int c4 = h3c[p][(i + 5) % 9], c5 = h3c[p][(i + 4) % 9];
+                // row r - 1: F = 9 y - S3
+                F[p] = 9 * yv[p][(i + 2) % 3] - (c2 + c1 + hc);
+                // row r - 2: GX = S3(x + 1) - S3(x - 1), GY = S3(y + 1) - S3(y - 1), zero on the picture's border
+                const int yg = r - 2;
+                GX[p] = (x > 0 && x < w - 1) ? (h3r[p][0] + h3r[p][1] + h3r[p][2]) - (h3l[p][0] + h3l[p][1] + h3l[p][2]) : 0;
+                GY[p] = (yg > 0 && yg < h - 1) ? (hc + c1) - (c3 + c4) : 0;
+                // row r - 4: M = 9 S3 - S9
+                s9[p] += h9v - h9[p][i];
+                M[p] = 9 * (c5 + c4 + c3) - s9[p];
+                h3c[p][i] = hc;
+                h9[p][i] = h9v;
+                h3l[p][i % 3] = hl;
+                h3r[p][i % 3] = hr;
+                yv[p][i % 3] = v[4];
+            }
+            buf ^= 1;
+            const int yf = r - 1, yg = r - 2, ym = r - 4;
+            if (yf >= y0 && yf < y1) {
+                accF[0] = __mul24(F[0], F[0]) + accF[0];
+                accF[1] = __mul24(F[1], F[1]) + accF[1];
+                accF[2] = __mul24(F[0], F[1]) + accF[2];
+                if (yf % 16 == 15) flush(accF, winF, slotF, yf);
+            }
+            if (yg >= y0 && yg < y1) {
+                accG[0] = __mul24(GX[0], GX[0]) + accG[0];
+                accG[1] = __mul24(GY[0], GY[0]) + accG[1];
+                accG[2] = __mul24(GX[0], GY[0]) + accG[2];
+                accG[3] = __mul24(GX[1], GX[1]) + accG[3];
+                accG[4] = __mul24(GY[1], GY[1]) + accG[4];
+                accG[5] = __mul24(GX[1], GY[1]) + accG[5];
+                if (yg % 16 == 15) flush(accG, winG, slotG, yg);
+            }
+            if (ym >= y0 && ym < y1) {
+#pragma unroll
+                for (int p = 0; p < 2; p++)
+                    if (mid[p] && inside) ((ES_GLOBAL int32_t *)mid[p])[(size_t)ym * w + x] = M[p];
+                accM[0] += (long long)M[0] * M[0];
+                accM[1] += (long long)M[1] * M[1];
+                accM[2] += (long long)M[0] * M[1];
+                if (ym % 16 == 15) flush(accM, winM, slotM, ym);
+            }
+        }
+    }
+}
+
 template <int WIN>
 static void launch_ws(const WsPair *pairs, int npairs, int max_w, int max_h, bool fp64, hipStream_t st)
 {
     const int wcov = max_w / WIN * WIN, wh = max_h / WIN;
-    const dim3 grid((wcov + 255) / 256, (wh + kWsRows - 1) / kWsRows, npairs);
-    if (fp64) hipLaunchKernelGGL((window_stats_kernel<WIN, double, double>), grid, dim3(256), 0, st, pairs);
-    else hipLaunchKernelGGL((window_stats_kernel<WIN, int, long long>), grid, dim3(256), 0, st, pairs);
+    if (fp64) {
+        const dim3 grid((wcov + 255) / 256, (wh + kWsRows - 1) / kWsRows, npairs);
+        hipLaunchKernelGGL((window_stats_kernel<WIN, double, double>), grid, dim3(256), 0, st, pairs);
+    } else {
+        const dim3 grid((wcov + 63) / 64, (wh * WIN + kWsStrip - 1) / kWsStrip, npairs);
+        hipLaunchKernelGGL(window_stats_int_kernel<WIN>, grid, dim3(64), 0, st, pairs);
+    }
 }
 
 void launch_window_stats(const WsPair *pairs, int npairs, int max_w, int max_h, int win, bool fp64, hipStream_t st)
