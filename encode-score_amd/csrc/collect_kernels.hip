@@ -1,12 +1,12 @@
 // Device collect: the per-window part of the reference's host collect (src/main.cpp:576-643) for the batched form, and
 // the order-statistic selects.
 //
-// collect: one block of 256 threads per (encode, frame).  It selects the structure threshold, numpy's linear 0.8
-// quantile of the source's oriented energies (src/main.cpp:279-289, :606-608), with an in-block radix select (8
-// passes of 8-bit digits over the FP64 bit patterns, which order like the non-negative values); then per window the
+// collect, per (encode, frame) job: the structure threshold, numpy's linear 0.8 quantile of the source's oriented
+// energies (src/main.cpp:279-289, :606-608), by a radix select in one block per job (8 passes of 8-bit digits over the
+// FP64 bit patterns, which order like the non-negative values); then, one block per chunk of 256 windows, the
 // structure mask, the histogram entries (src/stats.hpp:16) and the exact frame totals; and, for the centre frame six
 // back, the lasting-texture windows (src/main.cpp:630-641).  Frame totals of the oriented energy use the canonical
-// order: blocks of 256 consecutive windows added as a halving tree, the blocks in order.
+// order: each chunk of 256 consecutive windows added as a halving tree, the chunks in order.
 //
 // hist_quantiles: Histogram::quantile (src/stats.hpp:21-30) for many histograms, the bin where the running count
 // first exceeds q (n - 1): a block scan over the 20001 bins.
@@ -122,15 +122,163 @@ __device__ K block_min_above(const K *keys, long long n, K a)
     return m;
 }
 
-constexpr int kCollectThreads = 256;
+constexpr int kCollectThreads = 256;  // one chunk of 256 windows per block
+constexpr uint16_t kNoBin = 0xFFFF;   // a window that adds nothing to a histogram
+constexpr int kSelectThreads = 1024;
 
-// The lasting-texture windows of the centre frame six back (src/main.cpp:630-641), with exact [own, mine, count].
-__device__ void collect_lasting(const CollectJob &J)
+// Sums of a chunk's per-thread values over the block (exact integers), from the block's 4 waves.
+template <int N, int P>
+__device__ __forceinline__ void block_sum_ll(long long (&v)[N], long long (&part)[P][kCollectThreads / 64])
 {
-    if (!J.psrc) return;
-    const int n = J.windows, tid = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();  // an earlier call's readers are done with part
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const long long s = wave_sum(v[k]);
+        if (lane == 0) part[k][wave] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = part[k][0] + part[k][1] + part[k][2] + part[k][3];
+}
+
+// Keys for the structure threshold: the bit patterns of the source's oriented energies (non-negative doubles, which
+// order like their values).
+__global__ __launch_bounds__(kCollectThreads) void collect_keys_kernel(const CollectJob *jobs, CollectWork w)
+{
+    const CollectJob &J = jobs[blockIdx.y];
+    const int i = blockIdx.x * kCollectThreads + threadIdx.x;
+    if (i < J.windows)
+        w.keys[(size_t)blockIdx.y * w.max_windows + i] =
+            (unsigned long long)__double_as_longlong(oriented_r(J.sums + (size_t)i * 12));
+}
+
+// numpy's linear quantile at 0.8 of a job's keys (src/main.cpp:279-289): the key of rank lo = floor(0.8 (n - 1)) by
+// 8 passes of 8-bit digits, each pass counting into per-wave histograms, and the next larger key for lo + 1.
+__global__ __launch_bounds__(kSelectThreads) void collect_select_kernel(const CollectJob *jobs, CollectWork w)
+{
+    constexpr int kWaves = kSelectThreads / 64;
+    __shared__ unsigned hist[kWaves][256];
+    __shared__ unsigned long long rank_sh;
+    __shared__ int digit_sh;
+    const CollectJob &J = jobs[blockIdx.x];
+    const long long n = J.windows;
+    if (n <= 0) return;
+    const unsigned long long *keys = w.keys + (size_t)blockIdx.x * w.max_windows;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    const double pos = 0.8 * (double)(n - 1);
+    const long long lo = (long long)pos;
+    unsigned long long prefix = 0, r = (unsigned long long)lo;
+    unsigned eq = 0;
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        for (int i = tid & 63; i < 256; i += 64) hist[wave][i] = 0;
+        __syncthreads();
+#pragma unroll 4
+        for (long long i = tid; i < n; i += kSelectThreads) {
+            const unsigned long long key = keys[i];
+            if (shift == 56 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
+                atomicAdd(&hist[wave][(unsigned)(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid < 256) {
+            unsigned t = 0;
+#pragma unroll
+            for (int k = 0; k < kWaves; k++) t += hist[k][tid];
+            hist[0][tid] = t;
+        }
+        __syncthreads();
+        const int d = find_digit(hist[0], r, &rank_sh, &digit_sh);
+        r = rank_sh;
+        eq = hist[0][d];
+        prefix |= (unsigned long long)d << shift;
+        __syncthreads();
+    }
+    double q = __longlong_as_double((long long)prefix);
+    if (lo + 1 < n) {
+        unsigned long long kb = prefix;
+        if (r + 1 >= eq) kb = block_min_above(keys, n, prefix);  // no second copy of the key at rank lo + 1
+        const double a = q, b = __longlong_as_double((long long)kb);
+        q = __dadd_rn(a, __dmul_rn(b - a, pos - (double)lo));
+    }
+    if (tid == 0) *J.quantile = q;
+}
+
+// Per chunk of 256 windows: the window figures, the structure mask, the histogram entries, the chunk's exact
+// integer totals and its oriented-energy sums as a halving tree; the job's last chunk block adds the chunks in order.
+__global__ __launch_bounds__(kCollectThreads) void collect_apply_kernel(const CollectJob *jobs, CollectWork w)
+{
+    __shared__ double tree[4][kCollectThreads];
+    __shared__ long long part[12][kCollectThreads / 64];
+    const int job = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
+    const CollectJob &J = jobs[job];
+    const int n = J.windows, nchunks = (n + kCollectThreads - 1) / kCollectThreads;
+    if (chunk >= nchunks) return;  // uniform over the block
+    const double q = *J.quantile;
+    const int i = chunk * kCollectThreads + tid;
+    long long zi[12] = {0};
+    double o[4] = {0, 0, 0, 0};
+    bool m = false;
+    int bins[5] = {0, 0, 0, 0, 0};
+    if (i < n) {
+        const long long *z = J.sums + (size_t)i * 12;
+        long long zz[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) zz[k] = z[k];
+        double v[8];
+        window_values(zz, v);
+        m = v[6] >= q && v[6] > 0;
+        J.mask[i] = (uint8_t)m;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            zi[k] = zz[k];
+            zi[6 + k] = m ? zz[k] : 0;
+        }
+        o[0] = v[6];
+        o[1] = v[7];
+        o[2] = m ? v[6] : 0.0;
+        o[3] = m ? v[7] : 0.0;
+        if (m) {
+            const double er[3] = {v[0], v[1], v[6]}, ed[3] = {v[2], v[3], v[7]};
+#pragma unroll
+            for (int k = 0; k < 3; k++) bins[k] = hist_bin(100.0 * sqrt(ed[k] / fmax(er[k], 1e-9)));
+#pragma unroll
+            for (int k = 0; k < 2; k++) bins[3 + k] = hist_bin(100.0 * v[4 + k] / fmax(er[k], 1e-9));
+        }
+    }
+    if (i < n) {
+        uint16_t *bo = w.bins + ((size_t)job * w.max_windows + i) * 6;
+#pragma unroll
+        for (int k = 0; k < 5; k++) bo[k] = m ? (uint16_t)bins[k] : kNoBin;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) tree[k][tid] = o[k];
+    __syncthreads();
+    for (int s = kCollectThreads / 2; s > 0; s >>= 1) {
+        if (tid < s)
+#pragma unroll
+            for (int k = 0; k < 4; k++) tree[k][tid] += tree[k][tid + s];
+        __syncthreads();
+    }
+    block_sum_ll(zi, part);
+    const size_t c = (size_t)job * w.max_chunks + chunk;
+    if (tid < 12) w.cint[c * 12 + tid] = zi[tid];
+    if (tid < 4) w.cdbl[c * 4 + tid] = tree[tid][0];
+}
+
+// The lasting-texture windows of the centre frame six back (src/main.cpp:630-641), per chunk of 256 windows, with
+// exact [own, mine, count] totals added up by the job's last chunk block.
+__global__ __launch_bounds__(kCollectThreads) void collect_lasting_kernel(const CollectJob *jobs, CollectWork w)
+{
+    __shared__ long long part[3][kCollectThreads / 64];
+    const int job = blockIdx.y, chunk = blockIdx.x, tid = threadIdx.x;
+    const CollectJob &J = jobs[job];
+    const int n = J.windows, nchunks = (n + kCollectThreads - 1) / kCollectThreads;
+    if (!J.psrc || chunk >= nchunks) return;  // uniform over the block
+    const int i = chunk * kCollectThreads + tid;
     long long lz[3] = {0, 0, 0};
-    for (int i = tid; i < n; i += kCollectThreads) {
+    bool keep = false;
+    int bin = 0;
+    if (i < n) {
         long long i0, i1, iab, iaa, ibb;
         if (J.lasting_plain) {
             const long long *p = J.psrc + (size_t)i * 5;
@@ -151,124 +299,120 @@ __device__ void collect_lasting(const CollectJob &J)
         const double v2 = (double)iab / 236196.0, v3 = (double)iaa / 236196.0, v4 = (double)ibb / 236196.0;
         const double agree = v2 / sqrt(fmax(__dmul_rn(v3, v4), 1e-9));
         if (J.cmask[i] && agree > 0.5 && v0 > 0) {
-            lz[0] += i0;
-            lz[1] += i1;
-            lz[2] += 1;
-            atomicAdd(J.lhist + (size_t)5 * kHistBins + hist_bin(100.0 * v1 / v0), 1ull);
+            lz[0] = i0;
+            lz[1] = i1;
+            lz[2] = 1;
+            keep = true;
+            bin = hist_bin(100.0 * v1 / v0);
         }
     }
-    __shared__ long long part[3][kCollectThreads / 64];
-    const int lane = tid & 63, wave = tid >> 6;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const long long s = wave_sum(lz[k]);
-        if (lane == 0) part[k][wave] = s;
-    }
-    __syncthreads();
-    if (tid == 0)
-        for (int k = 0; k < 3; k++) {
-            long long s = 0;
-            for (int wv = 0; wv < kCollectThreads / 64; wv++) s += part[k][wv];
-            J.tot_lasting[k] = s;
-        }
+    if (i < n) w.bins[((size_t)job * w.max_windows + i) * 6 + 5] = keep ? (uint16_t)bin : kNoBin;
+    block_sum_ll(lz, part);
+    const size_t c = (size_t)job * w.max_chunks + chunk;
+    if (tid < 3) w.clast[c * 3 + tid] = lz[tid];
 }
 
-__global__ __launch_bounds__(kCollectThreads) void collect_kernel(const CollectJob *jobs, double *scratch, int max_windows,
-                                                                   int phase)
+// A job's totals from its chunks' partial results (written by the launches before): the integer parts in any order,
+// every thread taking some chunks; the oriented sums chunk by chunk in order, staged through LDS 256 chunks at a time
+// and added by one thread.
+__global__ __launch_bounds__(kCollectThreads) void collect_total_kernel(const CollectJob *jobs, CollectWork w)
 {
-    const CollectJob &J = jobs[blockIdx.x];
-    const int n = J.windows, tid = threadIdx.x;
-    if (n <= 0) return;  // an inactive job (an encode that has ended)
-    if (phase == 1) {
-        collect_lasting(J);
-        return;
+    __shared__ double stage[4][kCollectThreads];
+    __shared__ long long part[15][kCollectThreads / 64];
+    const int job = blockIdx.x, tid = threadIdx.x;
+    const CollectJob &J = jobs[job];
+    const int n = J.windows, nchunks = (n + kCollectThreads - 1) / kCollectThreads;
+    if (n <= 0) return;
+    const long long *ci = w.cint + (size_t)job * w.max_chunks * 12;
+    const long long *cl = w.clast + (size_t)job * w.max_chunks * 3;
+    const double *cd = w.cdbl + (size_t)job * w.max_chunks * 4;
+    const bool lasting = J.psrc != nullptr;
+    long long t[15] = {0};
+    for (int k = tid; k < nchunks; k += kCollectThreads) {
+#pragma unroll
+        for (int j = 0; j < 12; j++) t[j] += ci[(size_t)k * 12 + j];
+        if (lasting)
+#pragma unroll
+            for (int j = 0; j < 3; j++) t[12 + j] += cl[(size_t)k * 3 + j];
     }
-    unsigned long long *keys = (unsigned long long *)scratch + (size_t)blockIdx.x * max_windows;
-    for (int i = tid; i < n; i += kCollectThreads) keys[i] = (unsigned long long)__double_as_longlong(oriented_r(J.sums + (size_t)i * 12));
-    __threadfence_block();
-    __syncthreads();
-    // numpy's linear quantile of the oriented energies at 0.8
-    const double pos = 0.8 * (double)(n - 1);
-    const long long lo = (long long)pos;
-    unsigned long long eq, before;
-    const unsigned long long ka = block_select(keys, n, (unsigned long long)lo, &eq, &before);
-    double q = __longlong_as_double((long long)ka);
-    if (lo + 1 < n) {
-        unsigned long long kb = ka;
-        if (before + 1 >= eq) kb = block_min_above(keys, n, ka);
-        const double a = q, b = __longlong_as_double((long long)kb);
-        q = __dadd_rn(a, __dmul_rn(b - a, pos - (double)lo));
-    }
-    if (tid == 0 && J.quantile) *J.quantile = q;
-
-    __shared__ double tree[4][kCollectThreads];
-    __shared__ long long part[12][kCollectThreads / 64];
-    long long zi[12] = {0};
+    block_sum_ll(t, part);
+    if (tid < 12) J.tot_int[tid] = t[tid];
+    if (lasting && tid >= 12 && tid < 15) J.tot_lasting[tid - 12] = t[tid];
     double tot[4] = {0, 0, 0, 0};
-    for (int b = 0; b < n; b += kCollectThreads) {
-        const int i = b + tid;
-        double o[4] = {0, 0, 0, 0};
-        if (i < n) {
-            const long long *z = J.sums + (size_t)i * 12;
-            double v[8];
-            window_values(z, v);
-            const bool m = v[6] >= q && v[6] > 0;
-            J.mask[i] = (uint8_t)m;
-#pragma unroll
-            for (int k = 0; k < 6; k++) {
-                zi[k] += z[k];
-                if (m) zi[6 + k] += z[k];
-            }
-            o[0] = v[6];
-            o[1] = v[7];
-            o[2] = m ? v[6] : 0.0;
-            o[3] = m ? v[7] : 0.0;
-            if (m) {
-                const double er[3] = {v[0], v[1], v[6]}, ed[3] = {v[2], v[3], v[7]};
-#pragma unroll
-                for (int k = 0; k < 3; k++)
-                    atomicAdd(J.hist + (size_t)k * kHistBins + hist_bin(100.0 * sqrt(ed[k] / fmax(er[k], 1e-9))), 1ull);
-#pragma unroll
-                for (int k = 0; k < 2; k++)
-                    atomicAdd(J.hist + (size_t)(3 + k) * kHistBins + hist_bin(100.0 * v[4 + k] / fmax(er[k], 1e-9)), 1ull);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) tree[k][tid] = o[k];
+    for (int k0 = 0; k0 < nchunks; k0 += kCollectThreads) {
         __syncthreads();
-        for (int s = kCollectThreads / 2; s > 0; s >>= 1) {
-            if (tid < s)
+        if (k0 + tid < nchunks)
 #pragma unroll
-                for (int k = 0; k < 4; k++) tree[k][tid] += tree[k][tid + s];
-            __syncthreads();
-        }
+            for (int j = 0; j < 4; j++) stage[j][tid] = cd[(size_t)(k0 + tid) * 4 + j];
+        __syncthreads();
         if (tid == 0)
+            for (int k = 0; k < kCollectThreads && k0 + k < nchunks; k++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) tot[k] += tree[k][0];
-        __syncthreads();
+                for (int j = 0; j < 4; j++) tot[j] += stage[j][k];
     }
-    // block sums of the exact totals
-    const int lane = tid & 63, wave = tid >> 6;
+    if (tid == 0)
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
-        const long long s = wave_sum(zi[k]);
-        if (lane == 0) part[k][wave] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        for (int k = 0; k < 12; k++) {
-            long long s = 0;
-            for (int wv = 0; wv < kCollectThreads / 64; wv++) s += part[k][wv];
-            J.tot_int[k] = s;
-        }
-        for (int k = 0; k < 4; k++) J.tot_orient[k] = tot[k];
-    }
+        for (int j = 0; j < 4; j++) J.tot_orient[j] = tot[j];
 }
 
-void launch_collect(const CollectJob *jobs, int njobs, double *scratch, int max_windows, int phase, hipStream_t st)
+// The histogram entries of the round.  One block per (frame, encode, histogram) counts the frame's bins in LDS and
+// writes the counts out; then one block per range of bins and (encode, histogram) adds the frames' counts, frame by
+// frame in order, into the histogram of each frame's segment (a run of frames of one segment is added at once).  Each
+// bin of each histogram has one writer, so plain read-modify-writes add the counts exactly.
+constexpr int kHistThreads = 1024;
+__global__ __launch_bounds__(kHistThreads) void collect_hist_kernel(const CollectJob *jobs, CollectWork w)
 {
+    __shared__ unsigned cnt[kHistBins];
+    const int job = blockIdx.x / 6, k = blockIdx.x % 6, tid = threadIdx.x;
+    const CollectJob &J = jobs[job];
+    const int n = J.windows;
+    if (n <= 0 || (k == 5 && !J.psrc)) return;  // uniform over the block: nothing to count
+    for (int b = tid; b < kHistBins; b += kHistThreads) cnt[b] = 0;
+    __syncthreads();
+    const uint16_t *bi = w.bins + (size_t)job * w.max_windows * 6 + k;
+    for (int i = tid; i < n; i += kHistThreads) {
+        const unsigned b = bi[(size_t)i * 6];
+        if (b != kNoBin) atomicAdd(&cnt[b], 1u);
+    }
+    __syncthreads();
+    unsigned *out = w.fhist + ((size_t)job * 6 + k) * kHistBins;
+    for (int b = tid; b < kHistBins; b += kHistThreads) out[b] = cnt[b];
+}
+
+__global__ __launch_bounds__(256) void collect_merge_kernel(const CollectJob *jobs, int nframes, int nenc, CollectWork w)
+{
+    const int e = blockIdx.y / 6, k = blockIdx.y % 6;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= kHistBins) return;
+    unsigned long long *cur = nullptr, acc = 0;
+    for (int f = 0; f < nframes; f++) {
+        const int job = f * nenc + e;
+        const CollectJob &J = jobs[job];
+        if (J.windows <= 0 || (k == 5 && !J.psrc)) continue;  // uniform: this frame has no counts here
+        unsigned long long *target = (k < 5 ? J.hist : J.lhist) + (size_t)k * kHistBins;
+        if (target != cur) {
+            if (cur && acc) cur[b] += acc;
+            cur = target;
+            acc = 0;
+        }
+        acc += w.fhist[((size_t)job * 6 + k) * kHistBins + b];
+    }
+    if (cur && acc) cur[b] += acc;
+}
+
+void launch_collect(const CollectJob *jobs, int nframes, int nenc, const CollectWork &w, hipStream_t st)
+{
+    const int njobs = nframes * nenc;
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(collect_kernel, dim3(njobs), dim3(kCollectThreads), 0, st, jobs, scratch, max_windows, phase);
+    const dim3 grid(w.max_chunks, njobs);
+    hipLaunchKernelGGL(collect_keys_kernel, grid, dim3(kCollectThreads), 0, st, jobs, w);
+    hipLaunchKernelGGL(collect_select_kernel, dim3(njobs), dim3(kSelectThreads), 0, st, jobs, w);
+    hipLaunchKernelGGL(collect_apply_kernel, grid, dim3(kCollectThreads), 0, st, jobs, w);
+    hipLaunchKernelGGL(collect_lasting_kernel, grid, dim3(kCollectThreads), 0, st, jobs, w);
+    hipLaunchKernelGGL(collect_total_kernel, dim3(njobs), dim3(kCollectThreads), 0, st, jobs, w);
+    hipLaunchKernelGGL(collect_hist_kernel, dim3(njobs * 6), dim3(kHistThreads), 0, st, jobs, w);
+    hipLaunchKernelGGL(collect_merge_kernel, dim3((kHistBins + 255) / 256, nenc * 6), dim3(256), 0, st, jobs, nframes,
+                       nenc, w);
     hip_check(hipGetLastError(), "collect");
 }
 

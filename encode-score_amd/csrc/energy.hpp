@@ -66,7 +66,19 @@ struct CollectJob {
 };
 
 // phase 0: the frame part of every job; phase 1: the lasting texture (after every job's phase 0 wrote its mask).
-void launch_collect(const CollectJob *jobs, int njobs, double *scratch, int max_windows, int phase, hipStream_t st);
+// Working memory of the collect: the keys of the structure threshold, the per-chunk partial results (chunks of 256
+// windows) and the windows' histogram bins.
+struct CollectWork {
+    unsigned long long *keys;  // njobs x max_windows
+    long long *cint;           // njobs x max_chunks x 12
+    double *cdbl;              // njobs x max_chunks x 4
+    long long *clast;          // njobs x max_chunks x 3
+    uint16_t *bins;            // njobs x max_windows x 6: each window's bin in the 6 histograms, or 0xFFFF
+    unsigned *fhist;           // njobs x 6 x 20001: each frame's counts
+    int max_windows, max_chunks;
+};
+// jobs[f * nenc + e] is frame f of the round for encode e.
+void launch_collect(const CollectJob *jobs, int nframes, int nenc, const CollectWork &w, hipStream_t st);
 void launch_hist_quantiles(const unsigned long long *hists, int nhist, const double *qs, int nq, double *out,
                            unsigned long long *counts, hipStream_t st);
 void launch_select_u64(const unsigned long long *keys, long long n, long long rank, unsigned long long *out,

@@ -396,6 +396,17 @@ public:
         tot_last.alloc((size_t)B * E * 3 * 8);
         quant.alloc((size_t)B * E * 8);
         scratch.alloc((size_t)B * E * max_windows * 8);
+        {
+            const int chunks = (max_windows + 255) / 256;
+            cwork_int.alloc((size_t)B * E * chunks * 12 * 8);
+            cwork_dbl.alloc((size_t)B * E * chunks * 4 * 8);
+            cwork_last.alloc((size_t)B * E * chunks * 3 * 8);
+            cwork_bins.alloc((size_t)B * E * max_windows * 6 * 2);
+            cwork_fhist.alloc((size_t)B * E * 6 * kHistBins * 4);
+            cwork = CollectWork{scratch.as<unsigned long long>(), cwork_int.as<long long>(), cwork_dbl.as<double>(),
+                                cwork_last.as<long long>(), cwork_bins.as<uint16_t>(), cwork_fhist.as<unsigned>(),
+                                max_windows, chunks};
+        }
         // descriptors: device arrays, staged from pinned host memory per round (two staging buffers)
         desc_bytes = align(P * sizeof(CambiPic)) + align(((size_t)P * kScales + 1) * 4) + align((size_t)B * E * sizeof(WsPair)) +
                      align((size_t)C * sizeof(LastingJob)) + align((size_t)B * E * sizeof(CollectJob));
@@ -502,6 +513,8 @@ private:
     std::vector<CropDev> cd;
     std::vector<EncDev> ed;
     DevBuf tot_int, tot_or, tot_last, quant, scratch, desc, lout;
+    DevBuf cwork_int, cwork_dbl, cwork_last, cwork_bins, cwork_fhist;
+    CollectWork cwork{};
     size_t lout_words = 0;
     size_t desc_bytes, res_bytes;
     HostBuf stage_buf[2], res[2];
@@ -675,8 +688,7 @@ private:
         case 6: launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st); break;
         case 7: launch_lasting(dptr<LastingJob>(o_last()), C, max_w, max_h, cfg.win, st); break;
         case 8:
-            launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 0, st);
-            launch_collect(dptr<CollectJob>(o_col()), B * E, scratch.as<double>(), max_windows, 1, st);
+            launch_collect(dptr<CollectJob>(o_col()), B, E, cwork, st);
             break;
         case 9: {
             char *r = res[p].as<char>();
