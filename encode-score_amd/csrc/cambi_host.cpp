@@ -34,12 +34,8 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     segs_.alloc((size_t)ntiles * 4);
     sums_.alloc(kScales * 8);
     counts_.alloc(kScales * 4);
-    hist_.alloc(kScales * 512 * 4);
-    hsum_.alloc(kScales * 512 * 8);
-    ticket_.alloc(kScales * 4);
-    hip_check(hipMemset(hist_.p, 0, hist_.bytes), "clearing");
-    hip_check(hipMemset(hsum_.p, 0, hsum_.bytes), "clearing");
-    hip_check(hipMemset(ticket_.p, 0, ticket_.bytes), "clearing");
+    hist_.alloc((size_t)kScales * kTopkBlocks * 512 * 4);
+    hsum_.alloc((size_t)kScales * kTopkBlocks * 512 * 8);
     TopkJob jobs[kScales] = {};
     for (int s = 0; s < kScales; s++) {
         jobs[s].list = list_.as<float>() + loff_[s];
@@ -48,9 +44,8 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
         jobs[s].seg_cap = kCvTileW * tile_h_;
         jobs[s].count = counts_.as<unsigned>() + s;
         jobs[s].k = (unsigned long long)info_.k[s];
-        jobs[s].hist = hist_.as<unsigned>() + s * 512;
-        jobs[s].hsum = hsum_.as<unsigned long long>() + s * 512;
-        jobs[s].ticket = ticket_.as<unsigned>() + s;
+        jobs[s].hist = hist_.as<unsigned>() + (size_t)s * kTopkBlocks * 512;
+        jobs[s].hsum = hsum_.as<unsigned long long>() + (size_t)s * kTopkBlocks * 512;
     }
     jobs_.alloc(sizeof jobs);
     hip_check(hipMemcpy(jobs_.p, jobs, sizeof jobs, hipMemcpyHostToDevice), "top-k jobs");

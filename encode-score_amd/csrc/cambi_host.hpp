@@ -29,7 +29,7 @@ private:
     CambiInfo info_;
     bool keep_;
     int tile_h_, tiles_ = 0, base_tiles_ = 0;
-    DevBuf consts_, planes_, cvals_, list_, segs_, sums_, counts_, jobs_, hist_, hsum_, ticket_, pic_, first_, score_;
+    DevBuf consts_, planes_, cvals_, list_, segs_, sums_, counts_, jobs_, hist_, hsum_, pic_, first_, score_;
     long long loff_[kScales] = {};
     int toff_[kScales] = {};
     HostBuf stage_, results_;

@@ -539,20 +539,19 @@ void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslot
 // ---------------------------------------------------------------------------------------------------------------
 // Top-k select.  Positive c-values are at least 0.5, so their bit patterns less 0x3F000000 are 27-bit keys in value
 // order; three passes of 9-bit digits select the k-th largest.  Each pass counts (and sums exactly) the digit of the
-// keys that match the digits chosen so far; the last block to finish walks the digits from the top, as the reference's
-// host loop does (src/cambi.cpp:151-156), keeps the count and sum of the keys above the chosen digit, and clears the
-// histogram for the next pass.  Jobs with no more than k positive values are skipped: their pooling needs no select.
+// keys that match the digits chosen so far, several blocks per job each into its own slot; then one block per job
+// adds the slots and walks the digits from the top, as the reference's host loop does (src/cambi.cpp:151-156), keeping
+// the count and sum of the keys above the chosen digit.  Jobs with no more than k positive values are skipped: their
+// pooling needs no select.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr unsigned kKeyBase = 0x3F000000u;
 
-__global__ __launch_bounds__(256) void topk_pass_kernel(TopkJob *jobs, int pass)
+__global__ __launch_bounds__(256) void topk_count_kernel(TopkJob *jobs, int pass)
 {
-    TopkJob &J = jobs[blockIdx.y];
-    const unsigned n = *J.count;
-    if ((unsigned long long)n <= J.k) return;
+    const TopkJob &J = jobs[blockIdx.y];
+    if ((unsigned long long)*J.count <= J.k) return;
     __shared__ unsigned hist[512];
     __shared__ unsigned long long hs[512];
-    __shared__ int last;
     const int tid = threadIdx.x;
     for (int i = tid; i < 512; i += 256) {
         hist[i] = 0;
@@ -575,45 +574,57 @@ __global__ __launch_bounds__(256) void topk_pass_kernel(TopkJob *jobs, int pass)
         }
     }
     __syncthreads();
-    for (int d = tid; d < 512; d += 256)
-        if (hist[d]) {
-            atomicAdd(&J.hist[d], hist[d]);
-            atomicAdd(&J.hsum[d], hs[d]);
-        }
-    __threadfence();
+    // this block's counts and sums, in its own slot
+    unsigned *ho = J.hist + (size_t)blockIdx.x * 512;
+    unsigned long long *so = J.hsum + (size_t)blockIdx.x * 512;
+    for (int d = tid; d < 512; d += 256) {
+        ho[d] = hist[d];
+        so[d] = hs[d];
+    }
+}
+
+// One block per job: the digit counts and sums of the counting blocks added up, then the walk from the top digit.
+__global__ __launch_bounds__(512) void topk_walk_kernel(TopkJob *jobs, int pass, int blocks)
+{
+    TopkJob &J = jobs[blockIdx.x];
+    if ((unsigned long long)*J.count <= J.k) return;
+    __shared__ unsigned hist[512];
+    __shared__ unsigned long long hs[512];
+    const int d = threadIdx.x;
+    unsigned c = 0;
+    unsigned long long sm = 0;
+#pragma unroll 4
+    for (int b = 0; b < blocks; b++) {
+        c += J.hist[(size_t)b * 512 + d];
+        sm += J.hsum[(size_t)b * 512 + d];
+    }
+    hist[d] = c;
+    hs[d] = sm;
     __syncthreads();
-    if (tid == 0) last = atomicAdd(J.ticket, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (tid == 0) {
+    if (d == 0) {
+        const int shift = 18 - 9 * pass;
+        const unsigned prefix = pass ? J.prefix : 0u;
         const unsigned remaining = pass ? J.remaining : (unsigned)J.k;
         unsigned above = 0;
         unsigned long long above_fx = 0;
-        int d = 511;
-        for (; d > 0; d--) {
-            const unsigned c = __hip_atomic_load(&J.hist[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (above + c >= remaining) break;
-            above += c;
-            above_fx += __hip_atomic_load(&J.hsum[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int g = 511;
+        for (; g > 0; g--) {
+            if (above + hist[g] >= remaining) break;
+            above += hist[g];
+            above_fx += hs[g];
         }
         J.remaining = remaining - above;
-        J.prefix = prefix | ((unsigned)d << shift);
+        J.prefix = prefix | ((unsigned)g << shift);
         J.above = (pass ? J.above : 0ull) + above;
         J.above_fx = (pass ? J.above_fx : 0ull) + above_fx;
     }
-    __syncthreads();  // the walk has read the histogram before it is cleared
-    for (int i = tid; i < 512; i += 256) {
-        J.hist[i] = 0;
-        J.hsum[i] = 0;
-    }
-    if (tid == 0) *J.ticket = 0;
 }
 
-void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks_per_job, hipStream_t st)
+void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks, hipStream_t st)
 {
     if (njobs <= 0) return;
-    hipLaunchKernelGGL(topk_pass_kernel, dim3(blocks_per_job, njobs), dim3(256), 0, st, jobs, pass);
+    hipLaunchKernelGGL(topk_count_kernel, dim3(blocks, njobs), dim3(256), 0, st, jobs, pass);
+    hipLaunchKernelGGL(topk_walk_kernel, dim3(njobs), dim3(512), 0, st, jobs, pass, blocks);
     hip_check(hipGetLastError(), "top-k select");
 }
 

@@ -68,9 +68,8 @@ struct TopkJob {
     int nseg, seg_cap;
     const unsigned int *count;
     unsigned long long k;
-    unsigned int *hist;            // [512] counts of the current digit
-    unsigned long long *hsum;      // [512] exact sums per digit
-    unsigned int *ticket;          // blocks done in the current pass
+    unsigned int *hist;            // [blocks][512] each counting block's counts of the current digit
+    unsigned long long *hsum;      // [blocks][512] and its exact sums per digit
     unsigned int prefix, remaining;
     unsigned long long above, above_fx;
 };
@@ -99,7 +98,7 @@ void launch_cambi_scale(const CambiPic *pics, const long long *px_first, int npi
 // tile_first[nslots] (device) holds the total; total_tiles sizes the grid when not persistent.
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
                           int persistent, unsigned int *work_counter, int blocks, hipStream_t st);
-void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks_per_job, hipStream_t st);
+void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks, hipStream_t st);
 void launch_cambi_finish(const CambiPic *pics, TopkJob *jobs, int npics, double *scores, hipStream_t st);
 void launch_clear(unsigned long long *a, long long na, unsigned int *b, long long nb, hipStream_t st);
 int base_tiles(int w, int h);

@@ -19,6 +19,8 @@
 
 namespace es {
 
+static constexpr int kTopkBlocks = 16;  // counting blocks per top-k job of a round
+
 struct Crop {
     int w, h, cx, cy, windows, ww, wh;
     long len;  // frames while any of its encodes runs
@@ -325,12 +327,8 @@ public:
         segs.alloc(seg_off[P] * 4);
         sums.alloc((size_t)P * kScales * 8);
         counts.alloc((size_t)P * kScales * 4 + 64);  // + the work counter
-        thist.alloc((size_t)P * kScales * 512 * 4);
-        thsum.alloc((size_t)P * kScales * 512 * 8);
-        tticket.alloc((size_t)P * kScales * 4);
-        hip_check(hipMemset(thist.p, 0, thist.bytes), "clearing");
-        hip_check(hipMemset(thsum.p, 0, thsum.bytes), "clearing");
-        hip_check(hipMemset(tticket.p, 0, tticket.bytes), "clearing");
+        thist.alloc((size_t)P * kScales * kTopkBlocks * 512 * 4);
+        thsum.alloc((size_t)P * kScales * kTopkBlocks * 512 * 8);
         std::vector<TopkJob> jobs((size_t)P * kScales);
         for (int p = 0; p < P; p++) {
             const int kc = crop_of_role(p % roles);
@@ -344,9 +342,8 @@ public:
                 j.seg_cap = kCvTileW * tile_h;
                 j.count = counts.as<unsigned>() + (size_t)p * kScales + s;
                 j.k = (unsigned long long)ci.k[s];
-                j.hist = thist.as<unsigned>() + ((size_t)p * kScales + s) * 512;
-                j.hsum = thsum.as<unsigned long long>() + ((size_t)p * kScales + s) * 512;
-                j.ticket = tticket.as<unsigned>() + (size_t)p * kScales + s;
+                j.hist = thist.as<unsigned>() + ((size_t)p * kScales + s) * kTopkBlocks * 512;
+                j.hsum = thsum.as<unsigned long long>() + ((size_t)p * kScales + s) * kTopkBlocks * 512;
             }
         }
         topk.alloc(jobs.size() * sizeof(TopkJob));
@@ -509,7 +506,7 @@ private:
     };
     std::vector<Layout> lay;
     std::vector<size_t> slot_off, list_off, seg_off;
-    DevBuf consts, planes, list, segs, sums, counts, thist, thsum, tticket, topk, scores;
+    DevBuf consts, planes, list, segs, sums, counts, thist, thsum, topk, scores;
     std::vector<CropDev> cd;
     std::vector<EncDev> ed;
     DevBuf tot_int, tot_or, tot_last, quant, scratch, desc, lout;
@@ -682,7 +679,7 @@ private:
                                  cv_blocks, st);
             break;
         case 4:
-            for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, 16, st);
+            for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, kTopkBlocks, st);
             break;
         case 5: launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st); break;
         case 6: launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st); break;
