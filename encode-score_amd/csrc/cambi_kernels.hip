@@ -2,13 +2,13 @@
 //
 // The per-picture pipeline of Cambi::launch / Cambi::finish (src/cambi.cpp:88-184) as four kernel kinds:
 //   cambi_base     scale 0: the spatial mask of the unfiltered picture (src/kernels.cu:9-40) and the 3x3 mode filter
-//                  (src/kernels.cu:486-500) from one LDS-staged tile, packed as value | mask << 15;
+//                  (src/kernels.cu:486-500) by wave strips, packed as value | mask << 15;
 //   cambi_scale    scales 1-4: decimation of the previous packed scale (src/kernels.cu:42-53) and the mode filter;
 //   cambi_cvalues  c-values of every scale (src/kernels.cu:129-278): per 64-column tile one wave keeps the window's
 //                  column counts of the tile's relevant values in LDS and turns them into window counts with a
 //                  wave-wide prefix sum per row; exact sums of the positive c-values and a compact list of them;
-//   topk_pass      the k-th largest positive c-value (src/kernels.cu:71-95, src/cambi.cpp:137-170) by a 3-pass
-//                  9-bit-digit radix select done on the device, the last block of each pass walking the digits;
+//   topk           the k-th largest positive c-value (src/kernels.cu:71-95, src/cambi.cpp:137-170) by a 3-pass
+//                  9-bit-digit radix select done on the device, a walk kernel choosing each pass's digit;
 //   cambi_finish   the pooled scales and the score (src/cambi.cpp:172-184).
 #include "device_util.hpp"
 
@@ -49,9 +49,12 @@ constexpr int kBaseW = 64, kBaseRows = 64;
 
 int base_tiles(int w, int h) { return ((w + kBaseW - 1) / kBaseW) * ((h + kBaseRows - 1) / kBaseRows); }
 
-__global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
+// One strip.  read(xx, yy) loads the sample at the nearest position inside the picture; whether (xx, yy) itself is
+// inside is applied where the value is used (outside, it reads as -1), so that the loads of row r + 2 issued during
+// row r are first waited for during row r + 1.
+template <typename Read>
+__device__ __forceinline__ void base_strip(const CambiPic &pic, Read read)
 {
-    const CambiPic &pic = pics[blockIdx.y];
     const int w = pic.sw[0], h = pic.sh[0];
     const int lane = threadIdx.x & 63, strip = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int across = (w + kBaseW - 1) / kBaseW;
@@ -60,16 +63,20 @@ __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
     const int y1 = min(y0 + kBaseRows, h);
     const int xa = x0 - 32 + lane, xb = x0 + 32 + lane, x = x0 + lane;
     const int index = pic.consts->mask_index;
-    auto at = [&](int xx, int yy) -> int {
-        return (xx >= 0 && xx < w && yy >= 0 && yy < h) ? (int)base_value(pic.in, w, h, xx, yy) : -1;
-    };
-    // rows r - 3 .. r + 3 of the mask window end at r + 3; the flat bits of row r need row r + 1
-    int ca = at(xa, y0 - 3), cb = at(xb, y0 - 3), cc = at(x, y0 - 3);
+    // rows r - 3 .. r + 3 of the mask window end at r + 3; the flat bits of row r need row r + 1, which was read
+    // (issued) one row earlier
+    const bool ina = xa >= 0 && xa < w, inb = xb >= 0 && xb < w, inc = x < w;
+    auto at = [&](int v, bool in, int yy) { return in && yy >= 0 && yy < h ? v : -1; };
+    int ca = at(read(xa, y0 - 3), ina, y0 - 3), cb = at(read(xb, y0 - 3), inb, y0 - 3), cc = at(read(x, y0 - 3), inc, y0 - 3);
+    int qa = read(xa, y0 - 2), qb = read(xb, y0 - 2), qc = read(x, y0 - 2);
     int last[7] = {0, 0, 0, 0, 0, 0, 0}, sum = 0;
     int hm[3] = {0, 0, 0}, cv[3] = {0, 0, 0};  // horizontal modes and own values of rows r - 2 .. r
     int md[3] = {0, 0, 0};                      // modes of rows r - 3 .. r - 1 (written 3 rows back)
     for (int r = y0 - 3; r < y1 + 3; r++) {
-        const int na = at(xa, r + 1), nb = at(xb, r + 1), nc = at(x, r + 1);
+        const int na = at(qa, ina, r + 1), nb = at(qb, inb, r + 1), nc = at(qc, inc, r + 1);
+        qa = read(xa, r + 2);
+        qb = read(xb, r + 2);
+        qc = read(x, r + 2);
         // flat bits of row r over columns x0 - 32 .. x0 + 95
         const int ra0 = __shfl_down(ca, 1), rb = __shfl_down(cb, 1), b0 = __shfl(cb, 0);
         const int ra = lane == 63 ? b0 : ra0;
@@ -102,10 +109,30 @@ __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
         md[1] = md[2];
         md[2] = m;
         const int y = r - 3;  // the row whose 7x7 count is complete; its mode is md[0]
-        if (y >= y0 && y < y1 && x < w) pic.planes[(size_t)y * w + x] = (uint16_t)(md[0] | ((sum > index ? 1 : 0) << 15));
+        if (y >= y0 && y < y1 && x < w)
+            ((ES_GLOBAL uint16_t *)pic.planes)[(size_t)y * w + x] = (uint16_t)(md[0] | ((sum > index ? 1 : 0) << 15));
         ca = na;
         cb = nb;
         cc = nc;
+    }
+}
+
+__global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
+{
+    const CambiPic &pic = pics[blockIdx.y];
+    const int w = pic.sw[0], h = pic.sh[0];
+    const PlaneIn in = pic.in;
+    auto cx = [=](int xx) { return min(max(xx, 0), w - 1); };
+    auto cy = [=](int yy) { return (size_t)min(max(yy, 0), h - 1) * in.pitch; };
+    // one reader per plane format, chosen once per block
+    if (in.from8) {
+        base_strip(pic, [=](int xx, int yy) { return (int)base_value(in, w, h, cx(xx), min(max(yy, 0), h - 1)); });
+    } else if (in.bytes == 1) {
+        const ES_GLOBAL uint8_t *p = (const ES_GLOBAL uint8_t *)in.ptr;
+        base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); });
+    } else {
+        const ES_GLOBAL uint16_t *p = (const ES_GLOBAL uint16_t *)in.ptr;
+        base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); });
     }
 }
 
