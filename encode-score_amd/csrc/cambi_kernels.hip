@@ -144,39 +144,59 @@ void launch_cambi_base(const CambiPic *pics, const int * /*tile_first*/, int npi
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// Scales 1-4: decimate + mode filter of the previous packed scale; the mask bit comes along from (2x, 2y).
+// Scales 1-4: decimate + mode filter of the previous packed scale; the mask bit comes along from (2x, 2y).  One wave
+// per strip of 64 columns and kScaleRows rows: per row each lane gathers the decimated sample of its column (and the
+// edge lanes the one beyond the strip), the horizontal 3-mode takes its neighbours by lane shuffles, and the vertical
+// 3-mode of the row above is formed from the last three rows.  The next row's samples are loaded a row ahead.
 // ---------------------------------------------------------------------------------------------------------------
+constexpr int kScaleRows = 32;
+
+int scale_strips(int w, int h) { return ((w + 63) / 64) * ((h + kScaleRows - 1) / kScaleRows); }
+
 __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, int s)
 {
     const CambiPic &pic = pics[blockIdx.y];
-    const int w = pic.sw[s], h = pic.sh[s];
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= (long long)w * h) return;
-    const int x = (int)(i % w), y = (int)(i / w);
-    const int pw = pic.sw[s - 1];
-    const uint16_t *prev = pic.planes + pic.off[s - 1];
-    auto d = [&](int xx, int yy) -> uint16_t { return prev[(size_t)(2 * yy) * pw + 2 * xx] & 0x7FFF; };
-    const unsigned mask = prev[(size_t)(2 * y) * pw + 2 * x] >> 15;
-    uint16_t m;
-    if (y == 0 || y == h - 1) {
-        m = d(x, y);
-    } else {
-        uint16_t t[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int yy = y - 1 + q;
-            t[q] = (x == 0 || x == w - 1) ? d(x, yy) : mode3(d(x - 1, yy), d(x, yy), d(x + 1, yy));
-        }
-        m = mode3(t[0], t[1], t[2]);
+    const int w = pic.sw[s], h = pic.sh[s], pw = pic.sw[s - 1];
+    const int lane = threadIdx.x & 63, strip = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int across = (w + 63) / 64;
+    if (strip >= across * ((h + kScaleRows - 1) / kScaleRows)) return;  // uniform over the wave
+    const int x0 = (strip % across) * 64, y0 = (strip / across) * kScaleRows, y1 = min(y0 + kScaleRows, h);
+    const int x = x0 + lane;
+    const ES_GLOBAL uint16_t *prev = (const ES_GLOBAL uint16_t *)(pic.planes + pic.off[s - 1]);
+    ES_GLOBAL uint16_t *out = (ES_GLOBAL uint16_t *)(pic.planes + pic.off[s]);
+    // this lane's column and its halo column (lane 0: the column left of the strip, lane 63: the one right of it)
+    const int xc = 2 * min(x, w - 1), xh = 2 * (lane == 0 ? max(x0 - 1, 0) : min(x0 + 64, w - 1));
+    auto row = [&](int r) { return (size_t)(2 * min(max(r, 0), h - 1)) * pw; };
+    // the horizontal 3-mode of a row from its own and halo samples; the picture's first and last columns keep theirs
+    auto hmode = [&](unsigned c, unsigned hv) -> unsigned {
+        const unsigned v = c & 0x7FFFu, e = hv & 0x7FFFu;
+        const unsigned left = __shfl_up(v, 1), right = __shfl_down(v, 1);
+        const unsigned l = lane == 0 ? e : left, rt = lane == 63 ? e : right;
+        return (x == 0 || x >= w - 1) ? v : (unsigned)mode3((uint16_t)l, (uint16_t)v, (uint16_t)rt);
+    };
+    // rows y0 - 1 and y0 to start: t = horizontal modes, c = raw samples (value and mask bit)
+    unsigned c0 = prev[row(y0 - 1) + xc], e0 = prev[row(y0 - 1) + xh];
+    unsigned c1 = prev[row(y0) + xc], e1 = prev[row(y0) + xh];
+    unsigned t0 = hmode(c0, e0), t1 = hmode(c1, e1);
+    unsigned nc = prev[row(y0 + 1) + xc], ne = prev[row(y0 + 1) + xh];
+    for (int y = y0; y < y1; y++) {
+        // row y + 1 is in; load row y + 2 for the next step
+        const unsigned c2 = nc, e2 = ne;
+        nc = prev[row(y + 2) + xc];
+        ne = prev[row(y + 2) + xh];
+        const unsigned t2 = hmode(c2, e2);
+        const unsigned m = (y == 0 || y == h - 1) ? (c1 & 0x7FFFu) : (unsigned)mode3((uint16_t)t0, (uint16_t)t1, (uint16_t)t2);
+        if (x < w) out[(size_t)y * w + x] = (uint16_t)(m | (c1 & 0x8000u));
+        t0 = t1;
+        t1 = t2;
+        c1 = c2;
     }
-    pic.planes[pic.off[s] + i] = (uint16_t)(m | (mask << 15));
 }
 
-void launch_cambi_scale(const CambiPic *pics, const long long * /*px_first*/, int npics, long long max_px, int s,
-                        hipStream_t st)
+void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, hipStream_t st)
 {
     if (npics <= 0) return;
-    hipLaunchKernelGGL(cambi_scale_kernel, dim3((unsigned)((max_px + 255) / 256), npics), dim3(256), 0, st, pics, s);
+    hipLaunchKernelGGL(cambi_scale_kernel, dim3((unsigned)((max_strips + 3) / 4), npics), dim3(256), 0, st, pics, s);
     hip_check(hipGetLastError(), "cambi scale");
 }
 

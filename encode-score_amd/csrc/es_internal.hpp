@@ -93,8 +93,8 @@ inline long long cambi_list_layout(const int *sw, const int *sh, int tile_h, lon
 
 // Host-side launch wrappers (cambi_kernels.hip).
 void launch_cambi_base(const CambiPic *pics, const int *tile_first, int npics, int total_tiles, hipStream_t st);
-void launch_cambi_scale(const CambiPic *pics, const long long *px_first, int npics, long long total_px, int s,
-                        hipStream_t st);
+int scale_strips(int w, int h);
+void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, hipStream_t st);
 // tile_first[nslots] (device) holds the total; total_tiles sizes the grid when not persistent.
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
                           int persistent, unsigned int *work_counter, int blocks, hipStream_t st);

@@ -351,9 +351,13 @@ public:
         scores.alloc((size_t)P * 8);
         max_base_tiles = 0;
         max_px.assign(kScales, 0);
+        max_strips.assign(kScales, 0);
         for (const Crop &cr : crops) {
             max_base_tiles = std::max(max_base_tiles, base_tiles(cr.w, cr.h));
-            for (int s = 0; s < kScales; s++) max_px[s] = std::max(max_px[s], cr.ci.n[s]);
+            for (int s = 0; s < kScales; s++) {
+                max_px[s] = std::max(max_px[s], cr.ci.n[s]);
+                max_strips[s] = std::max(max_strips[s], scale_strips(cr.ci.sw[s], cr.ci.sh[s]));
+            }
             max_w = std::max(max_w, cr.w);
             max_h = std::max(max_h, cr.h);
             max_windows = std::max(max_windows, cr.windows);
@@ -500,6 +504,7 @@ private:
 
     int B, R, P, roles, tile_h, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
     std::vector<long long> max_px;
+    std::vector<int> max_strips;
     struct Layout {
         long long loff[kScales], cap;
         int toff[kScales], tiles;
@@ -672,7 +677,7 @@ private:
             break;
         case 1: launch_cambi_base(pics, nullptr, P, max_base_tiles, st); break;
         case 2:
-            for (int s = 1; s < kScales; s++) launch_cambi_scale(pics, nullptr, P, max_px[s], s, st);
+            for (int s = 1; s < kScales; s++) launch_cambi_scale(pics, P, max_strips[s], s, st);
             break;
         case 3:
             launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, 0, tile_h, 1, counts.as<unsigned>() + (size_t)P * kScales,
