@@ -73,23 +73,24 @@ def config(n, es, frames):
     raise SystemExit(f"no configuration {n}")
 
 
-STAGE_KERNEL = {"cambi_cvalues": "cambi_cvalues_kernel", "cambi_base": "cambi_base_kernel",
-                "window_stats": "void window_stats_kernel<16, int, long long>", "persist": "void lasting_kernel<16>"}
+STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{bmax}>", "cambi_base": "cambi_base_kernel",
+                "window_stats": "void window_stats_int_kernel<16>", "persist": "void lasting_kernel<16>"}
 TRAFFIC_FILES = {(2, 3840, 2160, 16): "hbm_traffic_3840x2160_mosaic_batch16.json"}
 
 
-def measured_traffic(cfg, w, h, batch, stage):
+def measured_traffic(cfg, w, h, batch, stage, bmax):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
     workload (scaled by the copy-kernel calibration recorded beside them), or (None, reason)."""
     name = TRAFFIC_FILES.get((cfg, w, h, batch))
+    kernel = STAGE_KERNEL[stage].format(bmax=bmax)
     if name is None:
         return None, "no FETCH_SIZE/WRITE_SIZE profile of this workload and batch"
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            rec = json.load(f)["kernels"][STAGE_KERNEL[stage]]
+            rec = json.load(f)["kernels"][kernel]
     except (OSError, KeyError, ValueError):
-        return None, f"{name} holds no entry for {STAGE_KERNEL[stage]}"
+        return None, f"{name} holds no entry for {kernel}"
     return int(rec["bytes"]), f"profiles/{name}"
 
 
@@ -124,6 +125,9 @@ def main():
     ap.add_argument("--pool", type=int, default=512, help="distinct frames generated per file at most")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", type=int, default=-1,
+                    help="run the autotuner before the timed region (1), not (0), or for configuration 5 only (-1)")
+    ap.add_argument("--tune-cache", default=None, help="JSON file to keep autotuner results in")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the calibration copy kernel over this many MiB (for HBM byte counters)")
     args = ap.parse_args()
@@ -179,9 +183,22 @@ def main():
         wall = time.perf_counter() - t0
         return wall, start.elapsed_ms(stop) / 1e3
 
+    # the autotuner, on the pool's frames, outside the timed region
+    tuned = None
+    batch, tile_h, cv_bmax, cv_waves = args.batch, args.tile_h, 0, 0
+    if args.tune == 1 or (args.tune == -1 and args.config == 5):
+        n_tune = min(pool, 96)
+        s_t = [src_frames[f].ptr for f in range(n_tune)]
+        e_t = [[enc_frames[i][f].ptr for i in range(len(encs))] for f in range(n_tune)]
+        t0 = time.perf_counter()
+        tuned = es.autotune(W, H, descs, s_t, e_t, src_bytes=sbytes, src_shift=sshift, cache=args.tune_cache)
+        tuned["seconds"] = round(time.perf_counter() - t0, 2)
+        batch, tile_h, cv_bmax, cv_waves = tuned["batch"], tuned["tile_h"], tuned["cv_bmax"], tuned["cv_waves"]
+        print(f"[bench] autotuner: {tuned}", file=sys.stderr, flush=True)
+
     # batched form (graph replay), with live per-stage events
-    bat = es.Scorer(es.ES_BATCHED, W, H, descs, batch=args.batch, tile_h=args.tile_h, use_graph=True, profile=True,
-                    src_bytes=sbytes, src_shift=sshift)
+    bat = es.Scorer(es.ES_BATCHED, W, H, descs, batch=batch, tile_h=tile_h, use_graph=True, profile=True,
+                    src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves)
     s, e = ptrs(0, args.warmup)
     bat.submit(s, e)
     bat.sync()
@@ -198,7 +215,7 @@ def main():
     # the dominant kernel's roofline: algorithmic bytes per launch over its mean launch time (live HIP events)
     n_src = W * H
     crops = {(d.w, d.h) for d, _ in encs}
-    per_round_frames = args.batch
+    per_round_frames = batch
     pix = {"src": sum(cw * ch for cw, ch in crops), "enc": sum(d.w * d.h for d, _ in encs)}
     scale_px = lambda n: n * (1 + 1 / 4 + 1 / 16 + 1 / 64 + 1 / 256)
     B = per_round_frames
@@ -216,7 +233,7 @@ def main():
     dominant = max(alg_bytes, key=lambda k: stage[k])
     mean_ms = stage[dominant] / launches
     achieved = alg_bytes[dominant] / (mean_ms / 1e3) / 1e9
-    traffic, traffic_src = measured_traffic(args.config, W, H, args.batch, dominant)
+    traffic, traffic_src = measured_traffic(args.config, W, H, batch, dominant, cv_bmax or 32)
     # the whole step's input bytes (each luma plane read once) against HBM
     b_in = sbytes * n_src + sum(2 * d.w * d.h for d, _ in encs)
     step_gbs = b_in * steps_per_s / 1e9
@@ -235,7 +252,9 @@ def main():
         "dtype": "u16",
         "data": "synthetic",
         "config": {"workload": workload, "config": args.config, "width": W, "height": H, "encodes": len(encs),
-                   "batch": args.batch, "tile_h": args.tile_h or 32, "form": "batched, integer energies, graph replay"},
+                   "batch": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32, "cv_waves": cv_waves or 16,
+                   "form": "batched, integer energies, graph replay"},
+        "autotuner": tuned,
         "encode_frames_per_s": round(encode_frames / wall, 3),
         "device_ms_per_step": round(dev * 1e3 / args.steps, 4),
         "hbm_frac_b_in": round(step_gbs / HBM_PEAK_GBS, 6),
@@ -293,7 +312,7 @@ def main():
         gpu_cam, _ = bat.cambi(0, min(cpu_frames, descs[0].length))
         result["cambi_bit_exact_vs_oracle"] = bool(np.array_equal(gpu_cam, np.array(cam[: len(gpu_cam)])))
         # the batched form's scores over the same sample against the oracle's (FP64 formulas on exact sums)
-        chk = es.Scorer(es.ES_BATCHED, W, H, [es.EncodeDesc(d0.w, d0.h, cpu_frames, d0.w, 2, 0)], batch=args.batch,
+        chk = es.Scorer(es.ES_BATCHED, W, H, [es.EncodeDesc(d0.w, d0.h, cpu_frames, d0.w, 2, 0)], batch=batch,
                         use_graph=True, src_bytes=sbytes, src_shift=sshift)
         s_, e_ = ptrs(0, cpu_frames)
         chk.submit(s_, [row[:1] for row in e_])

@@ -13,6 +13,7 @@ from .escore import (  # noqa: F401
     EsError,
     EncodeDesc,
     Scorer,
+    autotune,
     SynthDesc,
     cambi_params,
     copy_kernel,

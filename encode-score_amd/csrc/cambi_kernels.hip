@@ -218,16 +218,19 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // processed, the setup's loads go out eight rows at a time, and the c-value arithmetic is predicated rather than
 // branched.  Positive c-values go to the tile's own segment of the list at a wave-uniform offset.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int kCvMaxB = 32;
+constexpr int kCvMaxB = 64;  // the largest pass; kernels are built for 32 and 64
 constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
 constexpr int kCvCols = 130;    // prefix entries per number: columns 0 .. 128, padded to an even count
 
+// LDS of one wave for passes of up to BMAX numbers: 14.5 KB for 32, 26.7 KB for 64 (fewer passes over tiles with
+// many values, fewer waves per CU).
+template <int BMAX>
 struct CvShared {
-    uint32_t cc[kCvMaxB / 4][128];            // column counts: numbers 4q .. 4q + 3 of column c in cc[q][c], u8 each
-    uint16_t pp[kCvMaxB][kCvCols];            // prefix over columns of the column counts of number b: pp[b][c]
+    uint32_t cc[BMAX / 4][128];               // column counts: numbers 4q .. 4q + 3 of column c in cc[q][c], u8 each
+    uint16_t pp[BMAX][kCvCols];               // prefix over columns of the column counts of number b: pp[b][c]
     uint16_t rank[kMaxBand + 2 * kDiffs];     // number of value v at rank[v + 4], kNone if not relevant
 };
-static_assert(sizeof(uint16_t) * kCvMaxB * kCvCols >= 2 * kCvFlags, "flags live in pp during the setup");
+static_assert(sizeof(uint16_t) * 32 * kCvCols >= 2 * kCvFlags, "flags live in pp during the setup");
 // Column counts are at most 2 pad + 1 <= 65 (u8), and a prefix over 128 columns at most 128 * 65 < 2^16 (u16).
 static_assert(2 * kMaxPad + 1 < 256 && 128 * (2 * kMaxPad + 1) < 65536, "count widths");
 
@@ -253,12 +256,17 @@ __device__ __forceinline__ unsigned cv_at(const CvTile &t, unsigned xo, int y)
     return __builtin_amdgcn_raw_buffer_load_b16(t.rs, xo + (unsigned)(y * t.w * 2), 0, 0);
 }
 
+// A packed sample less (mask bit | band_base): the value less band_base when the sample is masked and not below the
+// band; an unmasked sample (or the 0 read outside the picture) wraps far above any band size.  So the sample is
+// active exactly when the result is below band_size.
+__device__ __forceinline__ unsigned cv_off(const CvTile &t, unsigned v) { return v - (0x8000u + (unsigned)t.band_base); }
+
 // Index into the rank table of a packed sample: value - band_base + kDiffs when it is active, else 0 (whose entry is
 // kNone: no value below the band is relevant).
 __device__ __forceinline__ unsigned cv_index(const CvTile &t, unsigned v)
 {
-    const unsigned u = (v & 0x7FFFu) - (unsigned)t.band_base;
-    return (v >> 15) && u < (unsigned)t.band_size ? u + kDiffs : 0u;
+    const unsigned u = cv_off(t, v);
+    return u < (unsigned)t.band_size ? u + kDiffs : 0u;
 }
 
 // RN(1/n) for 1 <= n <= 8192, the entry of the reference's reciprocal table: the hardware reciprocal (within 1 ulp)
@@ -276,12 +284,12 @@ __device__ __forceinline__ float recip_rn(unsigned n)
 // The value less band_base of an active (masked, in-band) packed sample, else kNone.
 __device__ __forceinline__ unsigned cv_decode(const CvTile &t, unsigned v)
 {
-    const unsigned u = (v & 0x7FFFu) - (unsigned)t.band_base;
-    return (v >> 15) && u < (unsigned)t.band_size ? u : kNone;
+    const unsigned u = cv_off(t, v);
+    return u < (unsigned)t.band_size ? u : kNone;
 }
 
-template <int B>
-__device__ void cv_passes(CvShared &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
+template <int B, typename Sh>
+__device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
 {
     constexpr int Q = B / 4, NC = B / 8;  // count words per column; chunks of 8 numbers (2 words, 4 pairs)
     const int lane = threadIdx.x, pad = t.pad, span = 2 * pad + 1;
@@ -459,7 +467,8 @@ __device__ int find_slot(const int *first, int n, int t)
     return lo;
 }
 
-__device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int tile_h)
+template <int BMAX>
+__device__ void cv_tile(CvShared<BMAX> &sh, const CambiPic &pic, int s, int local, int tile_h)
 {
     const CambiConsts &K = *pic.consts;
     CvTile t;
@@ -544,7 +553,8 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
     if (relevant > 0) {
         if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_fx, tcount);
         else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_fx, tcount);
-        else cv_passes<32>(sh, t, relevant, sum_fx, tcount);
+        else if (BMAX == 32 || relevant <= 32) cv_passes<32>(sh, t, relevant, sum_fx, tcount);
+        else cv_passes<(BMAX > 32 ? BMAX : 32)>(sh, t, relevant, sum_fx, tcount);
     }
     sum_fx = wave_sum(sum_fx);
     if (lane == 0) {
@@ -554,10 +564,11 @@ __device__ void cv_tile(CvShared &sh, const CambiPic &pic, int s, int local, int
     }
 }
 
+template <int BMAX>
 __global__ __launch_bounds__(64) void cambi_cvalues_kernel(const CambiPic *pics, const int *tile_first, int nslots,
                                                            int tile_h, int persistent, unsigned int *counter)
 {
-    __shared__ CvShared sh;
+    __shared__ CvShared<BMAX> sh;
     const int total = tile_first[nslots];
     for (int t = persistent ? 0 : (int)blockIdx.x;;) {
         if (persistent) {
@@ -574,12 +585,16 @@ __global__ __launch_bounds__(64) void cambi_cvalues_kernel(const CambiPic *pics,
 }
 
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, hipStream_t st)
+                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st)
 {
     if (!persistent && total_tiles <= 0) return;
     const int grid = persistent ? blocks : total_tiles;
-    hipLaunchKernelGGL(cambi_cvalues_kernel, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h, persistent,
-                       work_counter);
+    if (bmax == 64)
+        hipLaunchKernelGGL(cambi_cvalues_kernel<64>, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h,
+                           persistent, work_counter);
+    else
+        hipLaunchKernelGGL(cambi_cvalues_kernel<32>, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h,
+                           persistent, work_counter);
     hip_check(hipGetLastError(), "cambi c-values");
 }
 

@@ -15,6 +15,7 @@
 #include "stats_host.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <memory>
 
 namespace es {
@@ -285,7 +286,9 @@ public:
         tile_h = c.tile_h > 0 ? c.tile_h : 32;
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "device attributes");
-        cv_blocks = std::max(1, cus) * 16;
+        cv_blocks = std::max(1, cus) * (c.cv_waves > 0 ? c.cv_waves : 16);
+        require(c.cv_bmax == 0 || c.cv_bmax == 32 || c.cv_bmax == 64, "cv_bmax must be 32 or 64");
+        cv_bmax = c.cv_bmax ? c.cv_bmax : 32;
         const int C = (int)crops.size(), E = (int)enc.size();
         roles = C + E;
         P = B * roles;
@@ -502,7 +505,7 @@ private:
         int n = 0;
     };
 
-    int B, R, P, roles, tile_h, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
+    int B, R, P, roles, tile_h, cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
     std::vector<long long> max_px;
     std::vector<int> max_strips;
     struct Layout {
@@ -681,7 +684,7 @@ private:
             break;
         case 3:
             launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, 0, tile_h, 1, counts.as<unsigned>() + (size_t)P * kScales,
-                                 cv_blocks, st);
+                                 cv_blocks, cv_bmax, st);
             break;
         case 4:
             for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, kTopkBlocks, st);
@@ -906,6 +909,75 @@ int es_scorer_thresholds(es_scorer *s, int encode, long n, double *q)
         const EncodeState &e = b.enc[encode];
         require(n >= 0 && n <= (long)e.quantile.size(), "more frames than scored");
         std::copy(e.quantile.begin(), e.quantile.begin() + n, q);
+    });
+}
+
+// The autotuner: times the batched form on the caller's frames, one setting at a time (coordinate descent from the
+// defaults): c-value tile height, c-value LDS layout, persistent c-value waves per CU, then frames per batch.  Each
+// trial builds a scorer, runs one batch to warm up (graph capture included), then times two batches.
+int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *const *src, const void *const *enc,
+                       es_tune_result *out)
+{
+    return guard([&] {
+        ensure_init();
+        require(cfg && src && enc && out, "null argument");
+        es_scorer_config base = *cfg;
+        base.mode = ES_BATCHED;
+        base.use_graph = 1;
+        base.profile = 0;
+        es_tune_result best{base.batch ? base.batch : 16, base.tile_h ? base.tile_h : 32, base.cv_bmax ? base.cv_bmax : 32,
+                            base.cv_waves ? base.cv_waves : 16, 0.0, 0};
+        const int ne = base.n_encodes;
+        auto trial = [&](const es_tune_result &t) -> double {
+            es_scorer_config c = base;
+            c.batch = t.batch;
+            c.tile_h = t.tile_h;
+            c.cv_bmax = t.cv_bmax;
+            c.cv_waves = t.cv_waves;
+            const int need = 3 * t.batch;
+            if (nframes < need) return -1.0;  // too few frames for this batch
+            BatchedScorer sc(c);
+            hipStream_t st = nullptr;
+            hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "creating a stream");
+            std::vector<const void *> e((size_t)need * ne);
+            for (int f = 0; f < need; f++)
+                for (int k = 0; k < ne; k++) e[(size_t)f * ne + k] = enc[(size_t)f * ne + k];
+            sc.submit(t.batch, src, e.data(), st);
+            sc.sync();
+            hip_check(hipStreamSynchronize(st), "tuning");
+            const auto t0 = std::chrono::steady_clock::now();
+            sc.submit(2 * t.batch, src + t.batch, e.data() + (size_t)t.batch * ne, st);
+            sc.sync();
+            hip_check(hipStreamSynchronize(st), "tuning");
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            hip_check(hipStreamDestroy(st), "tuning");
+            out->trials++;
+            return ms / (2 * t.batch);
+        };
+        out->trials = 0;
+        best.ms_per_step = trial(best);
+        require(best.ms_per_step > 0, "too few frames to tune: give at least three batches");
+        auto try_set = [&](auto set, std::initializer_list<int> values) {
+            es_tune_result keep = best;
+            for (int v : values) {
+                es_tune_result t = keep;
+                set(t, v);
+                if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves)
+                    continue;
+                const double ms = trial(t);
+                if (ms > 0 && ms < best.ms_per_step) {
+                    best = t;
+                    best.ms_per_step = ms;
+                }
+            }
+        };
+        try_set([](es_tune_result &t, int v) { t.tile_h = v; }, {16, 24, 32, 48, 64});
+        try_set([](es_tune_result &t, int v) { t.cv_bmax = v; }, {32, 64});
+        try_set([](es_tune_result &t, int v) { t.cv_waves = v; }, {8, 12, 16, 24});
+        try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32});
+        const int trials = out->trials;
+        *out = best;
+        out->trials = trials;
     });
 }
 

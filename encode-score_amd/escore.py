@@ -46,7 +46,13 @@ class EncodeDesc(C.Structure):
 class ScorerConfig(C.Structure):
     _fields_ = [("mode", _i), ("src_w", _i), ("src_h", _i), ("src_pitch", _i), ("src_bytes", _i), ("src_shift", _i),
                 ("depth", _i), ("win", _i), ("per_segment", _i), ("window_base", _i), ("batch", _i), ("tile_h", _i),
-                ("use_graph", _i), ("n_encodes", _i), ("encodes", C.POINTER(EncodeDesc)), ("profile", _i)]
+                ("use_graph", _i), ("n_encodes", _i), ("encodes", C.POINTER(EncodeDesc)), ("profile", _i),
+                ("cv_bmax", _i), ("cv_waves", _i)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("batch", _i), ("tile_h", _i), ("cv_bmax", _i), ("cv_waves", _i), ("ms_per_step", C.c_double),
+                ("trials", _i)]
 
 
 class Segment(C.Structure):
@@ -104,6 +110,7 @@ for _n, _a in [
     ("es_select_rank_u64", [_vp, _sz, _sz, C.POINTER(C.c_uint64), _vp]),
     ("es_hist_quantiles", [_vp, _i, _vp, _i, _vp, _vp]),
     ("es_scorer_create", [C.POINTER(ScorerConfig), C.POINTER(_vp)]), ("es_scorer_destroy", [_vp]),
+    ("es_scorer_autotune", [C.POINTER(ScorerConfig), _i, _vp, _vp, C.POINTER(TuneResult)]),
     ("es_scorer_submit", [_vp, _i, _vp, _vp, _vp]), ("es_scorer_sync", [_vp]),
     ("es_scorer_frames", [_vp, C.POINTER(C.c_long)]), ("es_scorer_segments", [_vp, _i, C.POINTER(_i)]),
     ("es_scorer_segment", [_vp, _i, _i, C.POINTER(Segment), _vp]),
@@ -318,10 +325,12 @@ class Scorer:
     """The measure path of one source against its encodes (src/main.cpp's queue/collect)."""
 
     def __init__(self, mode, src_w, src_h, encodes, depth=10, win=16, per_segment=1440, window_base=65, batch=16,
-                 tile_h=0, use_graph=False, src_pitch=None, src_bytes=2, src_shift=0, profile=False):
+                 tile_h=0, use_graph=False, src_pitch=None, src_bytes=2, src_shift=0, profile=False, cv_bmax=0,
+                 cv_waves=0):
         self._encodes = (EncodeDesc * len(encodes))(*encodes)
         cfg = ScorerConfig(mode, src_w, src_h, src_pitch or src_w, src_bytes, src_shift, depth, win, per_segment,
-                           window_base, batch, tile_h, int(use_graph), len(encodes), self._encodes, int(profile))
+                           window_base, batch, tile_h, int(use_graph), len(encodes), self._encodes, int(profile),
+                           cv_bmax, cv_waves)
         p = _vp()
         check(lib.es_scorer_create(C.byref(cfg), C.byref(p)))
         self.ptr = p.value
@@ -392,3 +401,41 @@ def synth_frame_host(desc, frame, bytes=2, threads=8):
     out = np.empty((desc.h, desc.w), np.uint8 if bytes == 1 else np.uint16)
     check(lib.es_synth_frame_host(C.byref(desc), frame, out.ctypes.data, bytes, threads))
     return out
+
+
+def autotune(src_w, src_h, encodes, src_ptrs, enc_ptrs, depth=10, win=16, src_bytes=2, src_shift=0, cache=None):
+    """The library's autotuner (es_scorer_autotune) on the given device frames (len(src_ptrs) steps; enc_ptrs[i] holds
+    step i's encode planes): a dict of batch, tile_h, cv_bmax, cv_waves, ms_per_step and trials.  With `cache` (a
+    JSON file path) a result is stored under the workload's shape and reused."""
+    import json
+    key = json.dumps({"src": [src_w, src_h, src_bytes, src_shift], "enc": [[e.w, e.h, e.bytes, e.shift] for e in encodes],
+                      "depth": depth, "win": win, "device": _device_key()}, sort_keys=True)
+    store = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            store = json.load(f)
+        if key in store:
+            return dict(store[key], cached=True)
+    descs = (EncodeDesc * len(encodes))(*encodes)
+    cfg = ScorerConfig(ES_BATCHED, src_w, src_h, src_w, src_bytes, src_shift, depth, win, 1440, 65, 16, 0, 1,
+                       len(encodes), descs, 0, 0, 0)
+    n = len(src_ptrs)
+    src = (_vp * n)(*src_ptrs)
+    flat = [p for row in enc_ptrs for p in row]
+    enc = (_vp * len(flat))(*flat)
+    r = TuneResult()
+    check(lib.es_scorer_autotune(C.byref(cfg), n, src, enc, C.byref(r)))
+    out = {"batch": r.batch, "tile_h": r.tile_h, "cv_bmax": r.cv_bmax, "cv_waves": r.cv_waves,
+           "ms_per_step": r.ms_per_step, "trials": r.trials}
+    if cache:
+        store[key] = out
+        os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump(store, f, indent=1, sort_keys=True)
+    return dict(out, cached=False)
+
+
+def _device_key():
+    n = C.c_int()
+    check(lib.es_device_count(C.byref(n)))
+    return f"gfx950 x{n.value}"

@@ -158,6 +158,8 @@ typedef struct {
     int n_encodes;
     const es_encode_desc *encodes;
     int profile;      /* batched form: time each stage of every round with events (es_scorer_stage_ms) */
+    int cv_bmax;      /* c-value LDS layout: numbers per pass, 32 or 64 (0 = 32) */
+    int cv_waves;     /* batched form: persistent c-value waves per CU (0 = 16) */
 } es_scorer_config;
 
 /* One segment of one encode (Accum, src/stats.hpp:36-82, measure fields), with its histograms' quantiles and the
@@ -174,6 +176,18 @@ typedef struct {
 } es_segment;
 
 int es_scorer_create(const es_scorer_config *cfg, es_scorer **out);
+/* Settings the autotuner chose and the time per frame step it measured with them. */
+typedef struct {
+    int batch, tile_h, cv_bmax, cv_waves;
+    double ms_per_step;
+    int trials;
+} es_tune_result;
+/* Autotuner (no counterpart in the reference): times the batched form of cfg's workload on the caller's frames
+ * (nframes steps as for es_scorer_submit, at least three batches of the largest batch tried), trying the c-value
+ * tile height, the c-value LDS layout, the c-value waves per CU and the batch one after another, and returns the
+ * fastest.  Each trial allocates a scorer of its own. */
+int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *const *src, const void *const *enc,
+                       es_tune_result *out);
 int es_scorer_destroy(es_scorer *s);
 /* Queues `nframes` frame steps: src[i] is the source's device plane of step i (the full picture), enc[i * n_encodes
  * + e] encode e's (ignored once the encode has ended). Device planes must stay unchanged until es_scorer_sync. */

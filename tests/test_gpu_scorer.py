@@ -177,3 +177,43 @@ def test_batched_larger_windows_match_oracle(gpu, orc, win):
     ref = orc.accum_summary(run.segment(0))
     compare_segment(got, ref)
     assert np.array_equal(hist, ref[1])
+
+
+@pytest.mark.parametrize("bmax,tile_h,waves", [(64, 32, 16), (64, 16, 8), (32, 48, 24), (32, 64, 12)])
+def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves):
+    """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, waves per CU)
+    gives the oracle's results; textured 640x360 frames have tiles with more than 32 relevant values."""
+    n, w, h = 14, 640, 360
+    print(f"case layout bmax {bmax} tile_h {tile_h} waves {waves}", flush=True)
+    src, enc = frames_of(gpu, n, w=w, h=h, seed=41, full=(w, h))
+    run, cam, cam_src = oracle_run(orc, src, enc, 1440)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    sc = gpu.Scorer(gpu.ES_BATCHED, w, h, [gpu.EncodeDesc(w, h, n, w, 2, 0)], batch=8, use_graph=True, tile_h=tile_h,
+                    cv_bmax=bmax, cv_waves=waves)
+    sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
+    sc.sync()
+    a, b = sc.cambi(0, n)
+    assert np.array_equal(a, cam) and np.array_equal(b, cam_src)
+    got, hist = sc.segment(0, 0, with_hist=True)
+    ref = orc.accum_summary(run.segment(0))
+    compare_segment(got, ref)
+    assert np.array_equal(hist, ref[1])
+
+
+def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
+    n = 96  # three batches of the largest batch tried
+    src, enc = frames_of(gpu, n, seed=12)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    descs = [gpu.EncodeDesc(W, H, n, W, 2, 0)]
+    cache = str(tmp_path / "tune.json")
+    r = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
+    assert r["batch"] in (8, 16, 32) and r["tile_h"] in (16, 24, 32, 48, 64)
+    assert r["cv_bmax"] in (32, 64) and r["cv_waves"] in (8, 12, 16, 24)
+    assert r["ms_per_step"] > 0 and r["trials"] >= 10 and not r["cached"]
+    again = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
+    assert again["cached"] and {k: again[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")} == \
+        {k: r[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")}
+    with pytest.raises(gpu.EsError):  # too few frames for three batches
+        gpu.autotune(W, H, descs, [d.ptr for d in ds[:10]], [[d.ptr] for d in de[:10]])
