@@ -75,23 +75,26 @@ def config(n, es, frames):
 
 STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{bmax}>", "cambi_base": "cambi_base_kernel",
                 "window_stats": "void window_stats_int_kernel<16>", "persist": "void lasting_kernel<16>"}
-TRAFFIC_FILES = {(2, 3840, 2160, 16): "hbm_traffic_3840x2160_mosaic_batch16.json"}
+TRAFFIC_FILES = {(2, 3840, 2160): "hbm_traffic_3840x2160_mosaic.json"}
 
 
 def measured_traffic(cfg, w, h, batch, stage, bmax):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
-    workload (scaled by the copy-kernel calibration recorded beside them), or (None, reason)."""
-    name = TRAFFIC_FILES.get((cfg, w, h, batch))
+    workload (scaled by the copy-kernel calibration recorded beside them, and by batch / the profiled batch: every
+    kernel's traffic is per frame of the launch), or (None, reason)."""
+    name = TRAFFIC_FILES.get((cfg, w, h))
     kernel = STAGE_KERNEL[stage].format(bmax=bmax)
     if name is None:
-        return None, "no FETCH_SIZE/WRITE_SIZE profile of this workload and batch"
+        return None, "no FETCH_SIZE/WRITE_SIZE profile of this workload"
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            rec = json.load(f)["kernels"][kernel]
+            prof = json.load(f)
+        rec, frames = prof["kernels"][kernel], prof["frames_per_launch"]
     except (OSError, KeyError, ValueError):
         return None, f"{name} holds no entry for {kernel}"
-    return int(rec["bytes"]), f"profiles/{name}"
+    note = f"profiles/{name}" + ("" if frames == batch else f", scaled from {frames} to {batch} frames per launch")
+    return int(rec["bytes"] * batch / frames), note
 
 
 def host_cpu():
@@ -120,13 +123,14 @@ def main():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--tile-h", type=int, default=0)
+    ap.add_argument("--cv-bmax", type=int, default=0, help="c-value LDS layout, 32 or 64 (with --tune 0)")
+    ap.add_argument("--cv-waves", type=int, default=0, help="persistent c-value waves per CU (with --tune 0)")
     ap.add_argument("--plain-steps", type=int, default=48)
     ap.add_argument("--cpu-frames", type=int, default=-1, help="frames of the CPU baseline sample (-1: per config)")
     ap.add_argument("--pool", type=int, default=512, help="distinct frames generated per file at most")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tune", type=int, default=-1,
-                    help="run the autotuner before the timed region (1), not (0), or for configuration 5 only (-1)")
+    ap.add_argument("--tune", type=int, default=1, help="run the autotuner before the timed region (1) or not (0)")
     ap.add_argument("--tune-cache", default=None, help="JSON file to keep autotuner results in")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the calibration copy kernel over this many MiB (for HBM byte counters)")
@@ -185,8 +189,8 @@ def main():
 
     # the autotuner, on the pool's frames, outside the timed region
     tuned = None
-    batch, tile_h, cv_bmax, cv_waves = args.batch, args.tile_h, 0, 0
-    if args.tune == 1 or (args.tune == -1 and args.config == 5):
+    batch, tile_h, cv_bmax, cv_waves = args.batch, args.tile_h, args.cv_bmax, args.cv_waves
+    if args.tune:
         n_tune = min(pool, 96)
         s_t = [src_frames[f].ptr for f in range(n_tune)]
         e_t = [[enc_frames[i][f].ptr for i in range(len(encs))] for f in range(n_tune)]
