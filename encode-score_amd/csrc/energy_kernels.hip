@@ -368,6 +368,8 @@ __global__ __launch_bounds__(256) void persist_plain_kernel(const PersistJob *jo
 
 // Batched form: one wave per 16 x 16 patch of the area the windows cover, 4 pixels of a row per lane; the patch sums
 // of a centre go to the window directly (16 x 16 windows) or by 64-bit atomic adds into cleared outputs (larger ones).
+// Per centre a lane reads the source planes c - 7, c and c + 6 and the encodes' planes c; plane c - 1 is the previous
+// centre's plane c, kept in registers.
 template <int WIN>
 __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
 {
@@ -390,6 +392,13 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
             sb[k] = J.sb[i0 + k];
         }
     }
+    // the previous centre's plane, which is this centre's plane c - 1: read once, kept in registers
+    int m1[4] = {0, 0, 0, 0};
+    if (c0 > kRadius) {
+        const int32_t *p1 = plane(c0 - 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) m1[k] = p1[k];
+    }
     for (int ci = 0; ci < J.ncentres; ci++) {
         const int c = c0 + ci;
         int m0[4];
@@ -404,16 +413,18 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
                 m0[k] = p[6][k];
             }
         } else {
-            const int32_t *pc = plane(c), *p6 = plane(c + 6), *p7 = plane(c - 7), *p1 = plane(c - 1);
+            const int32_t *pc = plane(c), *p6 = plane(c + 6), *p7 = plane(c - 7);
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 m0[k] = pc[k];
                 const int a = sb[k] - pc[k] + p6[k];
-                const int b = sa[k] - p7[k] + p1[k];
+                const int b = sa[k] - p7[k] + m1[k];
                 sa[k] = a;
                 sb[k] = b;
             }
         }
+#pragma unroll
+        for (int k = 0; k < 4; k++) m1[k] = m0[k];
         long long v[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 4; k++) {

@@ -912,8 +912,8 @@ int es_scorer_thresholds(es_scorer *s, int encode, long n, double *q)
     });
 }
 
-// The autotuner: times the batched form on the caller's frames, one setting at a time (coordinate descent from the
-// defaults): c-value tile height, c-value LDS layout, persistent c-value waves per CU, then frames per batch.  Each
+// The autotuner: times the batched form on the caller's frames, one setting at a time from the defaults: the c-value
+// tile height and LDS layout together, then persistent c-value waves per CU, then frames per batch.  Each
 // trial builds a scorer, runs one batch to warm up (graph capture included), then times two batches.
 int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *const *src, const void *const *enc,
                        es_tune_result *out)
@@ -971,8 +971,9 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
                 }
             }
         };
-        try_set([](es_tune_result &t, int v) { t.tile_h = v; }, {16, 24, 32, 48, 64});
-        try_set([](es_tune_result &t, int v) { t.cv_bmax = v; }, {32, 64});
+        // tile height and LDS layout together: taller tiles see more values, which favours the wider passes
+        try_set([](es_tune_result &t, int v) { t.tile_h = v % 100, t.cv_bmax = v / 100; },
+                {3216, 3224, 3232, 3248, 3264, 6416, 6424, 6432, 6448, 6464});
         try_set([](es_tune_result &t, int v) { t.cv_waves = v; }, {8, 12, 16, 24});
         try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32});
         const int trials = out->trials;

@@ -184,7 +184,7 @@ typedef struct {
 } es_tune_result;
 /* Autotuner (no counterpart in the reference): times the batched form of cfg's workload on the caller's frames
  * (nframes steps as for es_scorer_submit, at least three batches of the largest batch tried), trying the c-value
- * tile height, the c-value LDS layout, the c-value waves per CU and the batch one after another, and returns the
+ * tile height with the c-value LDS layout (together), then the c-value waves per CU, then the batch, and returns the
  * fastest.  Each trial allocates a scorer of its own. */
 int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *const *src, const void *const *enc,
                        es_tune_result *out);

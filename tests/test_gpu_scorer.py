@@ -211,7 +211,7 @@ def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
     r = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
     assert r["batch"] in (8, 16, 32) and r["tile_h"] in (16, 24, 32, 48, 64)
     assert r["cv_bmax"] in (32, 64) and r["cv_waves"] in (8, 12, 16, 24)
-    assert r["ms_per_step"] > 0 and r["trials"] >= 10 and not r["cached"]
+    assert r["ms_per_step"] > 0 and r["trials"] >= 14 and not r["cached"]
     again = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
     assert again["cached"] and {k: again[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")} == \
         {k: r[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")}
