@@ -167,12 +167,22 @@ __global__ __launch_bounds__(256) void window_stats_kernel(const WsPair *pairs)
 constexpr int kWsStrip = 64;
 
 
-template <int WIN>
-__global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pairs)
+// A sample of a plane of BYTES bytes per sample (1 or 2), shifted to the pipeline depth.
+template <int BYTES>
+__device__ __forceinline__ int plane_sample(const PlaneIn &in, int x, int y)
+{
+    const size_t i = (size_t)y * in.pitch + x;
+    const unsigned v = BYTES == 1 ? ((const ES_GLOBAL uint8_t *)in.ptr)[i] : ((const ES_GLOBAL uint16_t *)in.ptr)[i];
+    return (int)(v << in.shift);
+}
+
+typedef int WsRows[2][2][72 + 64];  // [buffer][picture][column - (x0 - 4)]; 72 .. 135 take unused halo stores
+
+// One strip of the pair; BR and BD are the source's and the encode's bytes per sample.
+template <int WIN, int BR, int BD>
+__device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf)
 {
     static_assert(kWsStrip % WIN == 0 && WIN % 16 == 0, "windows tile the strip in 16-row blocks");
-    __shared__ int rowbuf[2][2][72 + 64];  // [buffer][picture][column - (x0 - 4)]; 72 .. 135 take unused halo stores
-    const WsPair P = pairs[blockIdx.z];  // a copy: the stores below cannot change it, so it is read once
     const int lane = threadIdx.x;
     const int w = P.w, h = P.h, ww = w / WIN, H = h / WIN * WIN, W = ww * WIN;
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kWsStrip;
@@ -186,7 +196,10 @@ __global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pair
     const int hslot = lane < 4 ? lane : lane < 8 ? lane + 64 : 72 + lane;
     const PlaneIn in[2] = {P.r, P.d};
     int32_t *mid[2] = {P.mid_r, P.mid_d};
-    auto sample = [&](int p, int xx, int r) { return in_sample(in[p], xx, min(max(r, 0), h - 1)) << P.up; };
+    auto sample = [&](int p, int xx, int r) {
+        const int yy = min(max(r, 0), h - 1);
+        return (p == 0 ? plane_sample<BR>(in[0], xx, yy) : plane_sample<BD>(in[1], xx, yy)) << P.up;
+    };
     int pr[2], ph[2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
@@ -296,6 +309,21 @@ __global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pair
                 if (ym % 16 == 15) flush(accM, winM, slotM, ym);
             }
         }
+    }
+}
+
+template <int WIN>
+__global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pairs)
+{
+    __shared__ WsRows rowbuf;
+    const WsPair P = pairs[blockIdx.z];  // a copy: the stores cannot change it, so it is read once
+    // one reader per pair of plane formats, chosen once per block
+    if (P.r.bytes == 1) {
+        if (P.d.bytes == 1) ws_strip<WIN, 1, 1>(P, rowbuf);
+        else ws_strip<WIN, 1, 2>(P, rowbuf);
+    } else {
+        if (P.d.bytes == 1) ws_strip<WIN, 2, 1>(P, rowbuf);
+        else ws_strip<WIN, 2, 2>(P, rowbuf);
     }
 }
 
