@@ -29,6 +29,11 @@ __device__ __forceinline__ unsigned base_value(const PlaneIn &in, int w, int h, 
     return raw_sample(in, x, y);
 }
 
+// Neighbour lanes across the whole wave by DPP (lane i takes lane i + 1, or lane i - 1); the lane with no neighbour
+// gets 0 and is replaced by its caller.
+__device__ __forceinline__ int from_next_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int from_prev_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
+
 // mode3 (src/kernels.cu:55-60)
 __device__ __forceinline__ uint16_t mode3(uint16_t a, uint16_t b, uint16_t c)
 {
@@ -78,7 +83,7 @@ __device__ __forceinline__ void base_strip(const CambiPic &pic, Read read)
         qb = read(xb, r + 2);
         qc = read(x, r + 2);
         // flat bits of row r over columns x0 - 32 .. x0 + 95
-        const int ra0 = __shfl_down(ca, 1), rb = __shfl_down(cb, 1), b0 = __shfl(cb, 0);
+        const int ra0 = from_next_lane(ca), rb = from_next_lane(cb), b0 = __builtin_amdgcn_readlane(cb, 0);
         const int ra = lane == 63 ? b0 : ra0;
         const bool fa = ca >= 0 && (xa == w - 1 || ca == ra) && (r == h - 1 || ca == na);
         const bool fb = cb >= 0 && (xb == w - 1 || cb == rb) && (r == h - 1 || cb == nb);
@@ -93,7 +98,8 @@ __device__ __forceinline__ void base_strip(const CambiPic &pic, Read read)
         last[6] = count;
         // horizontal mode of row r at column x; the first and last columns keep their own value
         // every lane takes part in every shuffle; the edge lanes then pick the margin columns' values
-        const int left = __shfl_up(cc, 1), right = __shfl_down(cc, 1), a31 = __shfl(ca, 31), b32 = __shfl(cb, 32);
+        const int left = from_prev_lane(cc), right = from_next_lane(cc);
+        const int a31 = __builtin_amdgcn_readlane(ca, 31), b32 = __builtin_amdgcn_readlane(cb, 32);
         const int lft = lane == 0 ? a31 : left, rgt = lane == 63 ? b32 : right;
         const int h3 = (x == 0 || x >= w - 1) ? cc : (int)mode3((uint16_t)lft, (uint16_t)cc, (uint16_t)rgt);
         hm[0] = hm[1];
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
     // the horizontal 3-mode of a row from its own and halo samples; the picture's first and last columns keep theirs
     auto hmode = [&](unsigned c, unsigned hv) -> unsigned {
         const unsigned v = c & 0x7FFFu, e = hv & 0x7FFFu;
-        const unsigned left = __shfl_up(v, 1), right = __shfl_down(v, 1);
+        const unsigned left = (unsigned)from_prev_lane((int)v), right = (unsigned)from_next_lane((int)v);
         const unsigned l = lane == 0 ? e : left, rt = lane == 63 ? e : right;
         return (x == 0 || x >= w - 1) ? v : (unsigned)mode3((uint16_t)l, (uint16_t)v, (uint16_t)rt);
     };
