@@ -1,8 +1,8 @@
 """Benchmark of the per-frame measure path on one MI355X.
 
 A step is one frame step: the source's frame and every active encode's frame through CAMBI (all scales, top-k), the
-window energies, the lasting texture and the collect.  Inputs are synthetic and generated into device memory before
-the timed region.  `value` is the batched form's frame steps per second (graph replay, integer energies); the plain
+window energies, the lasting texture, PSNR and the mean colours (frames with chroma planes, read in place) and the
+collect.  Inputs are synthetic and generated into device memory before the timed region.  `value` is the batched form's frame steps per second (graph replay, integer energies); the plain
 per-frame-launch FP64 form is timed on the same frames in the same process.  The CPU baseline is the oracle run on
 the host cores over a bounded sample of the same frames.
 
@@ -74,7 +74,8 @@ def config(n, es, frames):
 
 
 STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{bmax}>", "cambi_base": "cambi_base_kernel",
-                "window_stats": "void window_stats_int_kernel<16>", "persist": "void lasting_kernel<16>"}
+                "window_stats": "void window_stats_int_kernel<16>", "persist": "void lasting_kernel<16>",
+                "frame_sums": "frame_sums_kernel"}
 TRAFFIC_FILES = {(2, 3840, 2160): "hbm_traffic_3840x2160_mosaic.json"}
 
 
@@ -134,6 +135,10 @@ def main():
     ap.add_argument("--tune-cache", default=None, help="JSON file to keep autotuner results in")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the calibration copy kernel over this many MiB (for HBM byte counters)")
+    ap.add_argument("--chroma", type=int, default=1,
+                    help="frames carry chroma planes: PSNR and mean colours in every step (1) or luma only (0)")
+    ap.add_argument("--ingest", type=int, default=50,
+                    help="time this many launches of the frame-ingest kernel on a P010 frame of the source's size")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py runs on one GPU only: this path has no multi-GPU form here (asked for {args.gpus})", file=sys.stderr)
@@ -162,15 +167,44 @@ def main():
             es.copy_kernel(b, a, nbytes)
         es.synchronize()
         del a, b
+    def chroma_desc(desc, salt):
+        """A half-size picture of the same generator for a chroma plane."""
+        d = es.SynthDesc()
+        C_ = type(d)
+        for name, _ in C_._fields_:
+            setattr(d, name, getattr(desc, name))
+        d.w, d.h, d.full_w, d.full_h = desc.w // 2, desc.h // 2, desc.full_w // 2, desc.full_h // 2
+        d.crop_x, d.crop_y, d.salt = desc.crop_x // 2, desc.crop_y // 2, desc.salt + salt
+        return d
+
     src_frames = gen(src_d, sbytes)
     enc_frames = [gen(d, 2) for d, _ in encs]
+    src_chroma = [gen(chroma_desc(src_d, 100 * k), sbytes) for k in (1, 2)] if args.chroma else None
+    enc_chroma = [[gen(chroma_desc(d, 100 * k), 2) for k in (1, 2)] for d, _ in encs] if args.chroma else None
     es.synchronize()
     descs = [es.EncodeDesc(d.w, d.h, length, d.w, 2, 0) for d, length in encs]
+    src_fmt = "yuv420p" if sbytes == 1 else "yuv420p10"
 
     def ptrs(f0, n):
         s = [src_frames[f % pool].ptr for f in range(f0, f0 + n)]
         e = [[enc_frames[i][f % pool].ptr for i in range(len(encs))] for f in range(f0, f0 + n)]
         return s, e
+
+    def yuv(f0, n, encodes=None):
+        """Frames with chroma of steps f0 .. f0 + n - 1 (source, and rows of the chosen encodes)."""
+        idx = range(len(encs)) if encodes is None else encodes
+        s = [es.frame_view(src_fmt, src_frames[f % pool], src_chroma[0][f % pool], src_chroma[1][f % pool],
+                           W * sbytes, W // 2 * sbytes) for f in range(f0, f0 + n)]
+        e = [[es.frame_view("yuv420p10", enc_frames[i][f % pool], enc_chroma[i][0][f % pool],
+                            enc_chroma[i][1][f % pool], encs[i][0].w * 2, encs[i][0].w) for i in idx]
+             for f in range(f0, f0 + n)]
+        return s, e
+
+    def submit(scorer, f0, n, stream=None):
+        if args.chroma:
+            scorer.submit_frames(*yuv(f0, n), stream)
+        else:
+            scorer.submit(*ptrs(f0, n), stream)
 
     def timed(scorer, f0, n):
         es.synchronize()
@@ -178,8 +212,7 @@ def main():
         stream = es.Stream()
         t0 = time.perf_counter()
         start.record(stream)
-        s, e = ptrs(f0, n)
-        scorer.submit(s, e, stream)
+        submit(scorer, f0, n, stream)
         scorer.sync()
         stop.record(stream)
         stream.synchronize()
@@ -203,8 +236,7 @@ def main():
     # batched form (graph replay), with live per-stage events
     bat = es.Scorer(es.ES_BATCHED, W, H, descs, batch=batch, tile_h=tile_h, use_graph=True, profile=True,
                     src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves)
-    s, e = ptrs(0, args.warmup)
-    bat.submit(s, e)
+    submit(bat, 0, args.warmup)
     bat.sync()
     warm_stage, warm_rounds = bat.stage_ms()
     print(f"[bench] warmup done ({args.warmup} frames)", file=sys.stderr, flush=True)
@@ -232,14 +264,16 @@ def main():
         "window_stats": (sbytes * pix["src"] + 2 * pix["enc"] + 4 * (pix["src"] + pix["enc"])) * B,
         # the B + 13 source mid planes the round's centres reach, SA and SB in and out, one encode mid plane per centre
         "persist": 4 * ((B + 13 + 4) * pix["src"] + B * pix["enc"]),
+        # luma and both chroma planes of both pictures (4:2:0: 1.5 samples per pixel)
+        "frame_sums": 1.5 * (sbytes * pix["src"] + 2 * pix["enc"]) * B if args.chroma else 0,
     }
     launches = max(rounds, 1)
     dominant = max(alg_bytes, key=lambda k: stage[k])
     mean_ms = stage[dominant] / launches
     achieved = alg_bytes[dominant] / (mean_ms / 1e3) / 1e9
     traffic, traffic_src = measured_traffic(args.config, W, H, batch, dominant, cv_bmax or 32)
-    # the whole step's input bytes (each luma plane read once) against HBM
-    b_in = sbytes * n_src + sum(2 * d.w * d.h for d, _ in encs)
+    # the whole step's input bytes (each luma plane, and with chroma each chroma plane, read once) against HBM
+    b_in = (sbytes * n_src + sum(2 * d.w * d.h for d, _ in encs)) * (1.5 if args.chroma else 1.0)
     step_gbs = b_in * steps_per_s / 1e9
 
     result = {
@@ -257,7 +291,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "config": args.config, "width": W, "height": H, "encodes": len(encs),
                    "batch": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32, "cv_waves": cv_waves or 16,
-                   "form": "batched, integer energies, graph replay"},
+                   "chroma": bool(args.chroma), "form": "batched, integer energies, graph replay"},
         "autotuner": tuned,
         "encode_frames_per_s": round(encode_frames / wall, 3),
         "device_ms_per_step": round(dev * 1e3 / args.steps, 4),
@@ -274,8 +308,7 @@ def main():
         n_plain = min(args.plain_steps, args.steps)
         pl = es.Scorer(es.ES_PLAIN, W, H, [es.EncodeDesc(d.w, d.h, min(d.length, args.warmup + n_plain), d.pitch, 2, 0)
                                           for d in descs], src_bytes=sbytes, src_shift=sshift)
-        s, e = ptrs(0, args.warmup)
-        pl.submit(s, e)
+        submit(pl, 0, args.warmup)
         pl.sync()
         pwall, _ = timed(pl, args.warmup, n_plain)
         plain_sps = n_plain / pwall
@@ -290,6 +323,10 @@ def main():
             b, _ = pl.cambi(i, n)
             agree &= bool(np.array_equal(a, b))
         result["plain_equals_batched_cambi"] = agree
+        if args.chroma:
+            result["plain_equals_batched_psnr"] = all(
+                bool(np.array_equal(bat.psnr(i, min(descs[i].length, args.warmup + n_plain)),
+                                    pl.psnr(i, min(descs[i].length, args.warmup + n_plain)))) for i in range(len(encs)))
         print(f"[bench] plain: {plain_sps:.2f} steps/s", file=sys.stderr, flush=True)
 
     # CPU baseline: the oracle on the host cores, bounded sample of the same frames
@@ -301,11 +338,24 @@ def main():
         d0, _ = encs[0]
         cx, cy = ((W - d0.w) // 2) & ~1, ((H - d0.h) // 2) & ~1
         run = orc.Run(d0.w, d0.h, threads=threads)
-        frames = [(src_frames[f].host().astype(np.uint16) << sshift, enc_frames[0][f].host()) for f in range(cpu_frames)]
-        t0 = time.perf_counter()
-        cam = []
-        for s_, e_ in frames:
-            cam.append(run.frame(s_[cy:cy + d0.h, cx:cx + d0.w], e_)[0])
+        crop = lambda a, k: a[cy // k:cy // k + d0.h // k, cx // k:cx // k + d0.w // k]
+        cam, psnr_cpu = [], []
+        if args.chroma:
+            frames = [((crop(src_frames[f].host().astype(np.uint16) << sshift, 1),
+                        crop(src_chroma[0][f].host().astype(np.uint16) << sshift, 2),
+                        crop(src_chroma[1][f].host().astype(np.uint16) << sshift, 2)),
+                       (enc_frames[0][f].host(), enc_chroma[0][0][f].host(), enc_chroma[0][1][f].host()))
+                      for f in range(cpu_frames)]
+            t0 = time.perf_counter()
+            for s_, e_ in frames:
+                a_, _, p_ = run.frame_yuv(s_, e_)
+                cam.append(a_)
+                psnr_cpu.append(p_)
+        else:
+            frames = [(src_frames[f].host().astype(np.uint16) << sshift, enc_frames[0][f].host()) for f in range(cpu_frames)]
+            t0 = time.perf_counter()
+            for s_, e_ in frames:
+                cam.append(run.frame(crop(s_, 1), e_)[0])
         cpu_t = time.perf_counter() - t0
         # pairs per second of the first encode, scaled to frame steps of this configuration's encodes
         pair_s = cpu_frames / cpu_t
@@ -318,8 +368,11 @@ def main():
         # the batched form's scores over the same sample against the oracle's (FP64 formulas on exact sums)
         chk = es.Scorer(es.ES_BATCHED, W, H, [es.EncodeDesc(d0.w, d0.h, cpu_frames, d0.w, 2, 0)], batch=batch,
                         use_graph=True, src_bytes=sbytes, src_shift=sshift)
-        s_, e_ = ptrs(0, cpu_frames)
-        chk.submit(s_, [row[:1] for row in e_])
+        if args.chroma:
+            chk.submit_frames(*yuv(0, cpu_frames, encodes=[0]))
+        else:
+            s_, e_ = ptrs(0, cpu_frames)
+            chk.submit(s_, [row[:1] for row in e_])
         chk.sync()
         got, ref = chk.segment(0, -1), orc.accum_summary(run.merged())[0]
 
@@ -331,7 +384,38 @@ def main():
         result["score_err_vs_oracle"] = {k: err(got[k], ref[k]) for k in
                                          ("cambi", "cambi_source", "texture", "banding", "grain", "score")}
         result["score_err_vs_oracle"]["lasting_windows_equal"] = bool(got["n_lasting"] == ref["n_lasting"])
+        if args.chroma:
+            result["score_err_vs_oracle"]["psnr"] = max(err(a, b) for a, b in zip(got["psnr"], ref["psnr"]))
+            result["score_err_vs_oracle"]["color"] = max(
+                err(a, b) for a, b in zip(got["color_source"] + got["color_encode"], ref["color_source"] + ref["color_encode"]))
+            result["psnr_bit_exact_vs_oracle"] = bool(np.array_equal(chk.psnr(0, cpu_frames), np.array(psnr_cpu)))
         print(f"[bench] cpu: {pair_s:.3f} pair-frames/s on {threads} threads", file=sys.stderr, flush=True)
+
+    # frame ingest (the reference's frame_planes): a P010 frame of the source's size to packed planes
+    if args.ingest:
+        uv = es.DeviceArray((H // 2, W), np.uint16)
+        es.synth_frame(es.SynthDesc(kind=1, seed=7, full_w=W, full_h=H // 2, w=W, h=H // 2, depth=10), 0, uv)
+        y = src_frames[0] if sbytes == 2 else es.DeviceArray((H, W), np.uint16)
+        if sbytes != 2:
+            es.synth_frame(es.SynthDesc(kind=1, seed=7, full_w=W, full_h=H, w=W, h=H, depth=10), 0, y)
+        fr = es.frame_view("p010", y, uv, None, W * 2, W * 2)
+        oy, ou, ov = es.DeviceArray((H, W), np.uint16), es.DeviceArray((H // 2, W // 2), np.uint16), \
+            es.DeviceArray((H // 2, W // 2), np.uint16)
+        stream = es.Stream()
+        for _ in range(3):
+            es.frame_planes(fr, 0, 0, 10, oy, ou, ov, W, H, stream)
+        start, stop = es.Event(), es.Event()
+        start.record(stream)
+        for _ in range(args.ingest):
+            es.frame_planes(fr, 0, 0, 10, oy, ou, ov, W, H, stream)
+        stop.record(stream)
+        stream.synchronize()
+        us = start.elapsed_ms(stop) * 1e3 / args.ingest
+        nbytes = 1.5 * W * H * 4  # 4:2:0 16-bit samples in, packed 16-bit samples out
+        result["ingest"] = {"kernel": "frame_planes_kernel<2, 1>", "frame": f"P010 {W}x{H}", "us_per_frame": round(us, 3),
+                            "algorithmic_bytes": int(nbytes), "achieved_gbs": round(nbytes / us / 1e3, 1),
+                            "frac": round(nbytes / us / 1e3 / HBM_PEAK_GBS, 4)}
+        print(f"[bench] ingest: {us:.2f} us per frame", file=sys.stderr, flush=True)
 
     print(json.dumps(result))
 

@@ -73,12 +73,13 @@ CambiUnit::~CambiUnit()
     if (copied_) (void)hipEventDestroy(copied_);
 }
 
-void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStream_t caller)
+void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStream_t caller, int rshift)
 {
     require(img != nullptr, "null picture");
     require(bytes == 1 || bytes == 2, "samples must be 1 or 2 bytes");
     require(pitch >= info_.w, "pitch below the width");
     require(shift >= 0 && shift <= 8, "bad shift");
+    require(rshift >= 0 && rshift <= 8 && (bytes == 2 || rshift == 0), "bad right shift");
     require(!pending_, "launch while the previous picture is not finished");
     CambiPic *p = stage_.as<CambiPic>();
     std::memset(p, 0, sizeof *p);
@@ -87,6 +88,7 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
     p->in.bytes = bytes;
     p->in.shift = info_.depth == 8 ? 2 + shift : shift;
     p->in.from8 = info_.depth == 8;
+    p->in.rshift = rshift;
     for (int s = 0; s < kScales; s++) {
         p->sw[s] = info_.sw[s];
         p->sh[s] = info_.sh[s];

@@ -12,7 +12,8 @@ public:
     CambiUnit(const CambiUnit &) = delete;
     CambiUnit &operator=(const CambiUnit &) = delete;
     // Reads the plane in stream order on `caller`; everything after the first kernel runs on the own stream.
-    void launch(const void *img, int pitch, int bytes, int shift, hipStream_t caller);
+    // the picture is read in place: sample = (word >> rshift) << shift, at `bytes` per sample
+    void launch(const void *img, int pitch, int bytes, int shift, hipStream_t caller, int rshift = 0);
     double finish();
     void scales(es_cambi_scale out[kScales]) const;
     void plane(int s, uint16_t *host) const;

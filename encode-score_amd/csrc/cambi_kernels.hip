@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
         base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); });
     } else {
         const ES_GLOBAL uint16_t *p = (const ES_GLOBAL uint16_t *)in.ptr;
-        base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); });
+        base_strip(pic, [=](int xx, int yy) { return (int)(((unsigned)p[cy(yy) + cx(xx)] >> in.rshift) << in.shift); });
     }
 }
 

@@ -11,7 +11,8 @@ namespace es {
 __device__ __forceinline__ unsigned raw_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = in.bytes == 1 ? ((const ES_GLOBAL uint8_t *)in.ptr)[i] : ((const ES_GLOBAL uint16_t *)in.ptr)[i];
+    const unsigned v = in.bytes == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)in.ptr)[i]
+                                     : (unsigned)((const ES_GLOBAL uint16_t *)in.ptr)[i] >> in.rshift;
     return v << in.shift;
 }
 

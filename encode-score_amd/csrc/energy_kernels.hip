@@ -39,7 +39,8 @@ __device__ __forceinline__ void push(T (&a)[N], T v)
 __device__ __forceinline__ int in_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = in.bytes == 1 ? ((const ES_GLOBAL uint8_t *)in.ptr)[i] : ((const ES_GLOBAL uint16_t *)in.ptr)[i];
+    const unsigned v = in.bytes == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)in.ptr)[i]
+                                     : (unsigned)((const ES_GLOBAL uint16_t *)in.ptr)[i] >> in.rshift;
     return (int)(v << in.shift);
 }
 
@@ -172,7 +173,8 @@ template <int BYTES>
 __device__ __forceinline__ int plane_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = BYTES == 1 ? ((const ES_GLOBAL uint8_t *)in.ptr)[i] : ((const ES_GLOBAL uint16_t *)in.ptr)[i];
+    const unsigned v = BYTES == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)in.ptr)[i]
+                                  : (unsigned)((const ES_GLOBAL uint16_t *)in.ptr)[i] >> in.rshift;
     return (int)(v << in.shift);
 }
 

@@ -36,11 +36,13 @@ struct CambiConsts {
     int tvi[kDiffs], weight[kDiffs];
 };
 
-// A luma plane as the kernels read it: sample(x, y) = plane[y * pitch + x] << shift, `bytes` 1 or 2 per sample, and
-// with from8 the 8-bit anti-dithering mean of cambi.c applied after the shift.
+// A luma plane as the kernels read it: sample(x, y) = (plane[y * pitch + x] >> rshift) << shift, `bytes` 1 or 2 per
+// sample (rshift only with 2: 6 for P010's high-bit samples), and with from8 the 8-bit anti-dithering mean of cambi.c
+// applied after the shifts.
 struct PlaneIn {
     const void *ptr;
     int pitch, bytes, shift, from8;
+    int rshift;
 };
 
 // One picture for the CAMBI kernels.  The packed scale planes hold value | mask << 15 per pixel, scale s at

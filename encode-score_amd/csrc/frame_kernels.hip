@@ -1,0 +1,304 @@
+// Frame ingest, PSNR and colour reductions for gfx950.
+//
+// frame_planes (src/kernels.cu:282-310): a decoded 4:2:0 frame, planar or semi-planar, 8 or 16 bits per sample,
+// cropped and brought to the pipeline depth as packed luma and chroma planes.  One thread per 2x2 luma quad and its
+// chroma sample, the quad's two luma rows written as one 32-bit word each.
+//
+// plane_sse / color_sums (src/kernels.cu:453-484): the squared error of two planes and the colour sums of a pair of
+// chroma planes, in exact integers.  Each block adds its total to the output with one 64-bit integer atomic per
+// figure: integer sums do not depend on the order of the adds, so the result is the same on every run.  The colour
+// strength sqrtf(du^2 + dv^2) is rounded to single precision as in the reference, then added exactly as an integer in
+// units of 2^-23 (every strength of a nonzero square is at least 1, so its float value is a multiple of 2^-23).
+//
+// frame_sums: the scorer's form, every (source, encode) pair of a round in one launch, reading the decoded frames in
+// place at their crops and formats (no packed copies): luma and chroma squared errors and both colour sums.
+#include "device_util.hpp"
+#include "frame.hpp"
+
+#include <type_traits>
+
+namespace es {
+
+// 2^23 sqrt(s) rounded to single precision (round to nearest), as an exact integer, for 0 <= s < 2^24.  The hardware
+// square root may be one unit in the last place off; the candidate is corrected by comparing s with the squares of
+// the midpoints to its neighbours in exact integer arithmetic.  With f = F 2^(e - 150) (F the 24-bit significand, e
+// the biased exponent) the midpoints are (4F +- 2) 2^(e - 152), or (4F - 1) 2^(e - 152) below a power of two, so
+// s lies above the upper one when s 2^(304 - 2e) > (4F + 2)^2.  No square root of an integer lies on a midpoint.
+__device__ __forceinline__ unsigned long long strength_fixed(unsigned s)
+{
+    if (s == 0) return 0;
+    unsigned b = __float_as_uint(__builtin_sqrtf((float)s));
+    {
+        const unsigned e = b >> 23, F = (b & 0x7FFFFFu) | 0x800000u;
+        const unsigned long long t = (unsigned long long)s << (304 - 2 * e);
+        const unsigned long long up = 4ull * F + 2, dn = F == 0x800000u ? 4ull * F - 1 : 4ull * F - 2;
+        if (t > up * up) b++;
+        else if (t < dn * dn) b--;
+    }
+    const unsigned e = b >> 23, F = (b & 0x7FFFFFu) | 0x800000u;
+    return (unsigned long long)F << (e - 127);
+}
+
+// The colour figures of one chroma sample pair at the 10-bit scale.
+__device__ __forceinline__ void color_add(unsigned u, unsigned v, unsigned long long &su, unsigned long long &sv,
+                                          unsigned long long &ss)
+{
+    const int du = (int)u - 512, dv = (int)v - 512;
+    su += u;
+    sv += v;
+    ss += strength_fixed((unsigned)(du * du + dv * dv));
+}
+
+// Adds N per-thread sums of a 256-thread block to out[0 .. N) with one atomic per nonzero block total.
+template <int N>
+__device__ __forceinline__ void block_add(unsigned long long (&v)[N], unsigned long long *out)
+{
+    __shared__ unsigned long long part[4][N];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; k++) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < N; k++) part[wave][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < N) {
+        const unsigned long long s = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+        if (s) atomicAdd(out + threadIdx.x, s);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// frame_planes
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int kQuadsPerBlock = 256;
+
+template <int BYTES, int SEMI>
+__global__ __launch_bounds__(256) void frame_planes_kernel(es_frame f, int x0, int y0, int lshift, uint16_t *y16,
+                                                           uint16_t *uo, uint16_t *vo, int w, int h)
+{
+    using T = typename std::conditional<BYTES == 1, uint8_t, uint16_t>::type;
+    const int w2 = w >> 1;
+    const int qx = blockIdx.x * kQuadsPerBlock + threadIdx.x, qy = blockIdx.y;
+    if (qx >= w2) return;
+    const int x = 2 * qx, y = 2 * qy;
+    const unsigned rs = (unsigned)f.shift;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        const ES_GLOBAL T *row = (const ES_GLOBAL T *)((const char *)f.y + (size_t)(y0 + y + r) * f.ypitch) + x0 + x;
+        const unsigned a = ((unsigned)row[0] >> rs) << lshift, b = ((unsigned)row[1] >> rs) << lshift;
+        *(ES_GLOBAL uint32_t *)(y16 + (size_t)(y + r) * w + x) = a | b << 16;
+    }
+    const int cx = qx + (x0 >> 1), cy = qy + (y0 >> 1);
+    const ES_GLOBAL T *cu = (const ES_GLOBAL T *)((const char *)f.u + (size_t)cy * f.cpitch);
+    unsigned u, v;
+    if (SEMI) {
+        u = cu[2 * cx];
+        v = cu[2 * cx + 1];
+    } else {
+        const ES_GLOBAL T *cv = (const ES_GLOBAL T *)((const char *)f.v + (size_t)cy * f.cpitch);
+        u = cu[cx];
+        v = cv[cx];
+    }
+    const size_t ci = (size_t)qy * w2 + qx;
+    ((ES_GLOBAL uint16_t *)uo)[ci] = (uint16_t)((u >> rs) << lshift);
+    ((ES_GLOBAL uint16_t *)vo)[ci] = (uint16_t)((v >> rs) << lshift);
+}
+
+void launch_frame_planes(const es_frame &f, int x0, int y0, int lshift, uint16_t *y16, uint16_t *u, uint16_t *v, int w,
+                         int h, hipStream_t st)
+{
+    const dim3 grid((w / 2 + kQuadsPerBlock - 1) / kQuadsPerBlock, h / 2);
+    auto k = f.bytes == 1 ? (f.semi ? frame_planes_kernel<1, 1> : frame_planes_kernel<1, 0>)
+                          : (f.semi ? frame_planes_kernel<2, 1> : frame_planes_kernel<2, 0>);
+    hipLaunchKernelGGL(k, grid, dim3(kQuadsPerBlock), 0, st, f, x0, y0, lshift, y16, u, v, w, h);
+    hip_check(hipGetLastError(), "frame planes");
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// plane_sse and color_sums
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int kSumThreads = 256;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kSumMaxBlocks = 2048;
+
+static int sum_blocks(size_t n)
+{
+    const size_t per_block = (size_t)kSumThreads * 16;  // about 16 samples per thread
+    return (int)std::max<size_t>(1, std::min<size_t>((n + per_block - 1) / per_block, kSumMaxBlocks));
+}
+
+__device__ __forceinline__ unsigned sq_diff(unsigned a, unsigned b)
+{
+    const int d = (int)a - (int)b;
+    return (unsigned)d * (unsigned)d;  // below 2^32 for 16-bit samples
+}
+
+// vec: a and b (or u and v) are 16-byte aligned; then eight samples per load, the remainder one at a time.
+template <bool VEC>
+__global__ __launch_bounds__(kSumThreads) void plane_sse_kernel(const uint16_t *a, const uint16_t *b, const float *fa,
+                                                                const float *fb, size_t n, unsigned long long *sse)
+{
+    unsigned long long s[1] = {0};
+    const size_t tid = (size_t)blockIdx.x * kSumThreads + threadIdx.x, stride = (size_t)gridDim.x * kSumThreads;
+    if (a) {
+        size_t i0 = 0;
+        if (VEC) {
+            const size_t n8 = n / 8;
+            const ES_GLOBAL u32x4 *a8 = (const ES_GLOBAL u32x4 *)a, *b8 = (const ES_GLOBAL u32x4 *)b;
+            for (size_t i = tid; i < n8; i += stride) {
+                const u32x4 x = a8[i], y = b8[i];
+                const unsigned xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    s[0] += (unsigned long long)sq_diff(xs[k] & 0xFFFFu, ys[k] & 0xFFFFu) + sq_diff(xs[k] >> 16, ys[k] >> 16);
+            }
+            i0 = n8 * 8;
+        }
+        for (size_t i = i0 + tid; i < n; i += stride) s[0] += sq_diff(((const ES_GLOBAL uint16_t *)a)[i], ((const ES_GLOBAL uint16_t *)b)[i]);
+    } else {
+        for (size_t i = tid; i < n; i += stride) {
+            const long long d = (long long)((const ES_GLOBAL float *)fa)[i] - (long long)((const ES_GLOBAL float *)fb)[i];
+            s[0] += (unsigned long long)(d * d);
+        }
+    }
+    block_add<1>(s, sse);
+}
+
+template <bool VEC>
+__global__ __launch_bounds__(kSumThreads) void color_sums_kernel(const uint16_t *u, const uint16_t *v, size_t n, int up,
+                                                                 unsigned long long *sums)
+{
+    unsigned long long s[3] = {0, 0, 0};
+    const size_t tid = (size_t)blockIdx.x * kSumThreads + threadIdx.x, stride = (size_t)gridDim.x * kSumThreads;
+    size_t i0 = 0;
+    if (VEC) {
+        const size_t n8 = n / 8;
+        const ES_GLOBAL u32x4 *u8 = (const ES_GLOBAL u32x4 *)u, *v8 = (const ES_GLOBAL u32x4 *)v;
+        for (size_t i = tid; i < n8; i += stride) {
+            const u32x4 x = u8[i], y = v8[i];
+            const unsigned xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                color_add((xs[k] & 0xFFFFu) << up, (ys[k] & 0xFFFFu) << up, s[0], s[1], s[2]);
+                color_add((xs[k] >> 16) << up, (ys[k] >> 16) << up, s[0], s[1], s[2]);
+            }
+        }
+        i0 = n8 * 8;
+    }
+    for (size_t i = i0 + tid; i < n; i += stride)
+        color_add((unsigned)((const ES_GLOBAL uint16_t *)u)[i] << up, (unsigned)((const ES_GLOBAL uint16_t *)v)[i] << up,
+                  s[0], s[1], s[2]);
+    block_add<3>(s, sums);
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+void launch_plane_sse(const uint16_t *a, const uint16_t *b, const float *fa, const float *fb, size_t n,
+                      unsigned long long *sse, hipStream_t st)
+{
+    if (n == 0) return;
+    const bool vec = a && aligned16(a) && aligned16(b);
+    hipLaunchKernelGGL(vec ? plane_sse_kernel<true> : plane_sse_kernel<false>, dim3(sum_blocks(n)), dim3(kSumThreads), 0,
+                       st, a, b, fa, fb, n, sse);
+    hip_check(hipGetLastError(), "plane sse");
+}
+
+void launch_color_sums(const uint16_t *u, const uint16_t *v, size_t n, int up, unsigned long long *sums, hipStream_t st)
+{
+    if (n == 0) return;
+    const bool vec = aligned16(u) && aligned16(v);
+    hipLaunchKernelGGL(vec ? color_sums_kernel<true> : color_sums_kernel<false>, dim3(sum_blocks(n)), dim3(kSumThreads),
+                       0, st, u, v, n, up, sums);
+    hip_check(hipGetLastError(), "color sums");
+}
+
+// ---------------------------------------------------------------------------------------------------------------
+// frame_sums: per block kFrameSumRows luma rows and half as many chroma rows of one job.  Each lane reads kSumUnroll
+// columns 256 apart before using any (clamped to the row, the surplus ones counted as zero), so a wave keeps many
+// loads in flight; the sample sizes of the two pictures are template parameters, chosen once per block.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int kSumUnroll = 4;
+
+template <int BYTES>
+__device__ __forceinline__ unsigned luma_at(const PlaneIn &in, int x, int y)
+{
+    const size_t i = (size_t)y * in.pitch + x;
+    const unsigned v = BYTES == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)in.ptr)[i]
+                                  : (unsigned)((const ES_GLOBAL uint16_t *)in.ptr)[i] >> in.rshift;
+    return v << in.shift;
+}
+
+template <int BYTES>
+__device__ __forceinline__ unsigned chroma_at(const ChromaIn &c, int x, int y)
+{
+    const size_t i = (size_t)y * c.pitch + (size_t)x * c.step;
+    const unsigned v = BYTES == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)c.ptr)[i]
+                                  : (unsigned)((const ES_GLOBAL uint16_t *)c.ptr)[i] >> c.rshift;
+    return v << c.shift;
+}
+
+template <int BR, int BD>
+__device__ __forceinline__ void frame_sums_block(const FrameSumJob &J, int r0)
+{
+    constexpr int U = kSumUnroll;
+    const int w = J.w, r1 = min(r0 + kFrameSumRows, J.h);
+    unsigned long long s[kFrameSums] = {};
+    for (int y = r0; y < r1; y++)
+        for (int xb = threadIdx.x; xb < w; xb += 256 * U) {
+            unsigned a[U], b[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int x = min(xb + 256 * k, w - 1);
+                a[k] = luma_at<BR>(J.ry, x, y);
+                b[k] = luma_at<BD>(J.dy, x, y);
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++) s[0] += xb + 256 * k < w ? sq_diff(a[k], b[k]) : 0u;
+        }
+    const int w2 = w >> 1, c0 = r0 >> 1, c1 = r1 >> 1, up = J.up;
+    const bool src_color = J.src_color;
+    for (int y = c0; y < c1; y++)
+        for (int xb = threadIdx.x; xb < w2; xb += 256 * U) {
+            unsigned ur[U], vr[U], ud[U], vd[U];
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                const int x = min(xb + 256 * k, w2 - 1);
+                ur[k] = chroma_at<BR>(J.ru, x, y);
+                vr[k] = chroma_at<BR>(J.rv, x, y);
+                ud[k] = chroma_at<BD>(J.du, x, y);
+                vd[k] = chroma_at<BD>(J.dv, x, y);
+            }
+#pragma unroll
+            for (int k = 0; k < U; k++) {
+                if (xb + 256 * k >= w2) break;  // the lane's columns are in order: the rest are past the row too
+                s[1] += sq_diff(ur[k], ud[k]);
+                s[2] += sq_diff(vr[k], vd[k]);
+                color_add(ud[k] << up, vd[k] << up, s[3], s[4], s[5]);
+                if (src_color) color_add(ur[k] << up, vr[k] << up, s[6], s[7], s[8]);
+            }
+        }
+    block_add<kFrameSums>(s, J.out);
+}
+
+__global__ __launch_bounds__(256) void frame_sums_kernel(const FrameSumJob *jobs)
+{
+    const FrameSumJob &J = jobs[blockIdx.y];
+    const int r0 = blockIdx.x * kFrameSumRows;
+    if (J.w <= 0 || r0 >= J.h) return;  // inactive job or past its rows: uniform over the block
+    if (J.ry.bytes == 2) {
+        if (J.dy.bytes == 2) frame_sums_block<2, 2>(J, r0);
+        else frame_sums_block<2, 1>(J, r0);
+    } else {
+        if (J.dy.bytes == 2) frame_sums_block<1, 2>(J, r0);
+        else frame_sums_block<1, 1>(J, r0);
+    }
+}
+
+void launch_frame_sums(const FrameSumJob *jobs, int njobs, int max_h, hipStream_t st)
+{
+    if (njobs <= 0 || max_h <= 0) return;
+    hipLaunchKernelGGL(frame_sums_kernel, dim3((max_h + kFrameSumRows - 1) / kFrameSumRows, njobs), dim3(256), 0, st, jobs);
+    hip_check(hipGetLastError(), "frame sums");
+}
+
+}  // namespace es

@@ -1,0 +1,20 @@
+// The run's reports in the reference's formats: the JSON file (src/main.cpp:693-777) and the text summary
+// (src/main.cpp:695-727), from the scorer's per-segment figures and per-frame lists.
+#pragma once
+#include "encscore.h"
+
+#include <cstdio>
+#include <vector>
+
+namespace es {
+
+struct ReportEncode {
+    es_segment whole;                  // the segments merged (Accum::merge)
+    std::vector<es_segment> segments;  // the segments that have frames, in order
+    std::vector<double> vmaf, cambi, cambi_source, psnr_y;  // per frame; vmaf may be shorter (not attached)
+};
+
+void write_json(FILE *out, const es_run_info &info, long frames, const std::vector<ReportEncode> &encs);
+void write_summary(FILE *out, const es_run_info &info, double seconds, long frames, const std::vector<ReportEncode> &encs);
+
+}  // namespace es

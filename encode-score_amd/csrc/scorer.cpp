@@ -9,9 +9,14 @@
 // source side of the lasting texture once per crop, and the collect and selects on the device; the round is captured
 // once as a HIP graph per staging buffer and replayed.  Per-pair crops and uneven encode lengths are handled by
 // descriptors: an encode that has ended leaves its pictures and jobs inactive.
+//
+// Frames are read in place in their decoded formats (es_frame: P010, NV12 and planar 4:2:0 at 8 or 16 bits per
+// sample); with chroma planes each frame step also measures PSNR and the mean colours (src/main.cpp:543-556).
 #include "cambi_host.hpp"
 #include "energy.hpp"
+#include "frame.hpp"
 #include "host_common.hpp"
+#include "report.hpp"
 #include "stats_host.hpp"
 
 #include <algorithm>
@@ -33,6 +38,8 @@ struct EncodeState {
     int crop, windows;
     std::vector<Accum> seg;
     std::vector<double> cambi, cambi_src, quantile;
+    std::vector<double> psnr;  // 3 per frame
+    std::vector<double> vmaf;  // attached by the caller
 };
 
 class ScorerBase {
@@ -80,9 +87,26 @@ public:
         up = 10 - c.depth;
     }
     virtual ~ScorerBase() = default;
-    virtual void submit(int n, const void *const *src, const void *const *encp, hipStream_t st) = 0;
+    virtual void submit(int n, const es_frame *src, const es_frame *encp, hipStream_t st) = 0;
     virtual void sync() = 0;
     virtual void segment_hist(int e, int seg, Accum &a) { (void)e, (void)seg, (void)a; }
+
+    // Segment k of encode e with its histograms and the attached VMAF of its frames.
+    Accum seg_accum(int e, int k)
+    {
+        Accum a = enc[e].seg[k];
+        segment_hist(e, k, a);
+        const std::vector<double> &v = enc[e].vmaf;
+        const long f0 = (long)k * cfg.per_segment, f1 = std::min<long>(f0 + cfg.per_segment, (long)v.size());
+        for (long f = f0; f < f1; f++) a.add_vmaf(v[f]);
+        return a;
+    }
+    // The segments of encode e that have scored frames.
+    int scored_segments(int e) const
+    {
+        const long n = (long)enc[e].cambi.size();
+        return (int)((n + cfg.per_segment - 1) / cfg.per_segment);
+    }
 
     es_scorer_config cfg;
     std::vector<es_encode_desc> encodes_copy;
@@ -90,6 +114,56 @@ public:
     std::vector<EncodeState> enc;
     int window_base, up;
     long frames = 0;  // frame steps submitted
+    int chroma = -1;  // whether the run's frames carry chroma planes (-1 before the first frame)
+
+    // The frames of a run as plain luma planes of the config's formats (es_scorer_submit).
+    void luma_frames(int n, const void *const *src, const void *const *encp, std::vector<es_frame> &fs,
+                     std::vector<es_frame> &fe) const
+    {
+        const int E = (int)enc.size();
+        fs.assign(n, es_frame{});
+        fe.assign((size_t)n * E, es_frame{});
+        for (int i = 0; i < n; i++) {
+            es_frame &f = fs[i];
+            f.y = src[i];
+            f.ypitch = cfg.src_pitch * cfg.src_bytes;
+            f.bytes = cfg.src_bytes;
+            f.depth = cfg.depth - cfg.src_shift;
+            for (int e = 0; e < E; e++) {
+                const es_encode_desc &d = enc[e].d;
+                es_frame &g = fe[(size_t)i * E + e];
+                g.y = encp[(size_t)i * E + e];
+                g.ypitch = d.pitch * d.bytes;
+                g.bytes = d.bytes;
+                g.depth = cfg.depth - d.shift;
+            }
+        }
+    }
+
+    // Checks the frames of steps frames .. frames + n - 1 and fixes whether the run carries chroma.
+    void check_frames(int n, const es_frame *src, const es_frame *encp)
+    {
+        const int E = (int)enc.size();
+        for (int i = 0; i < n; i++) {
+            const long f = frames + i;
+            const bool c = src[i].u != nullptr;
+            if (chroma < 0) {
+                chroma = c;
+                if (c)
+                    for (const Crop &cr : crops) require(cr.w % 2 == 0 && cr.h % 2 == 0, "picture sizes must be even with chroma");
+            }
+            require(c == (chroma == 1), "every frame of a run must carry chroma planes, or none");
+            check_frame(src[i], cfg.depth, c);
+            require(src[i].ypitch / src[i].bytes >= cfg.src_w, "source pitch below its width");
+            for (int e = 0; e < E; e++) {
+                if (!active(e, f)) continue;
+                const es_frame &g = encp[(size_t)i * E + e];
+                require((g.u != nullptr) == c, "every frame of a run must carry chroma planes, or none");
+                check_frame(g, cfg.depth, c);
+                require(g.ypitch / g.bytes >= enc[e].d.w, "encode pitch below its width");
+            }
+        }
+    }
 
 protected:
     bool active(int e, long f) const { return f < enc[e].d.length; }
@@ -99,17 +173,51 @@ protected:
             if (enc[e].crop == c && active((int)e, f)) return true;
         return false;
     }
-    PlaneIn src_plane(const void *src, int c) const
+    PlaneIn src_plane(const es_frame &f, int c) const { return frame_luma(f, crops[c].cx, crops[c].cy, cfg.depth); }
+    PlaneIn enc_plane(const es_frame &f) const { return frame_luma(f, 0, 0, cfg.depth); }
+    // The PSNR and colour job of encode e's frame pair; src_color: this job also sums the source's colour.
+    FrameSumJob frame_job(const es_frame &src, const es_frame &g, int e, bool src_color, unsigned long long *out) const
     {
-        const Crop &cr = crops[c];
-        const char *p = (const char *)src + ((size_t)cr.cy * cfg.src_pitch + cr.cx) * cfg.src_bytes;
-        return PlaneIn{p, cfg.src_pitch, cfg.src_bytes, cfg.src_shift, 0};
+        const Crop &cr = crops[enc[e].crop];
+        FrameSumJob j{};
+        j.ry = src_plane(src, enc[e].crop);
+        j.dy = enc_plane(g);
+        j.ru = frame_chroma(src, 0, cr.cx, cr.cy, cfg.depth);
+        j.rv = frame_chroma(src, 1, cr.cx, cr.cy, cfg.depth);
+        j.du = frame_chroma(g, 0, 0, 0, cfg.depth);
+        j.dv = frame_chroma(g, 1, 0, 0, cfg.depth);
+        j.w = cr.w;
+        j.h = cr.h;
+        j.up = up;
+        j.src_color = src_color;
+        j.out = out;
+        return j;
     }
-    PlaneIn enc_plane(const void *p, int e) const
+    // The encode whose job sums the source's colour of crop c in frame f: the first active encode of the crop.
+    int color_encode(int c, long f) const
     {
-        const es_encode_desc &d = enc[e].d;
-        return PlaneIn{p, d.pitch, d.bytes, d.shift, 0};
+        for (size_t e = 0; e < enc.size(); e++)
+            if (enc[e].crop == c && active((int)e, f)) return (int)e;
+        return -1;
     }
+    // Folds a frame's PSNR and colour sums (kFrameSums of them; the source's colour from its crop's first encode).
+    void fold_chroma(int e, long f, const unsigned long long *z, const unsigned long long *zs)
+    {
+        const Crop &cr = crops[enc[e].crop];
+        const size_t n = (size_t)cr.w * cr.h, nc = n / 4;
+        double p[3], col_r[3], col_d[3];
+        for (int k = 0; k < 3; k++) {
+            const double mse = (double)z[k] / (double)(k ? nc : n), peak = (double)((1 << cfg.depth) - 1);
+            p[k] = std::min(10.0 * std::log10(peak * peak / std::max(mse, 1e-16)), 6.0 * cfg.depth + 12);  // src/main.cpp:273-277
+            const double sd = k < 2 ? (double)z[3 + k] : std::ldexp((double)z[5], -23);
+            const double sr = k < 2 ? (double)zs[6 + k] : std::ldexp((double)zs[8], -23);
+            col_d[k] = sd / (double)nc;
+            col_r[k] = sr / (double)nc;
+        }
+        segment(e, f).add_chroma(p, col_r, col_d);
+        enc[e].psnr.insert(enc[e].psnr.end(), p, p + 3);
+    }
+    void no_chroma(int e) { enc[e].psnr.insert(enc[e].psnr.end(), 3, NAN); }
     Accum &segment(int e, long f) { return enc[e].seg[(size_t)(f / cfg.per_segment)]; }
 };
 
@@ -138,6 +246,8 @@ public:
                 b.psums[k].alloc((size_t)cr.windows * 5 * 8);
                 b.hsums[k].alloc((size_t)cr.windows * 12 * 8);
                 b.hpsums[k].alloc((size_t)cr.windows * 5 * 8);
+                b.fs[k].alloc(kFrameSums * 8);
+                b.hfs[k].alloc(kFrameSums * 8);
             }
             b.masks.assign(kRadius + 1, std::vector<uint8_t>(cr.windows, 0));
             eb.push_back(std::move(b));
@@ -150,8 +260,9 @@ public:
             if (e) (void)hipEventDestroy(e);
     }
 
-    void submit(int n, const void *const *src, const void *const *encp, hipStream_t st) override
+    void submit(int n, const es_frame *src, const es_frame *encp, hipStream_t st) override
     {
+        check_frames(n, src, encp);
         for (int i = 0; i < n; i++) {
             queue(frames, src[i], encp + (size_t)i * enc.size(), st);
             if (frames > 0) collect(frames - 1);
@@ -174,8 +285,8 @@ private:
     struct EncBufs {
         std::unique_ptr<CambiUnit> cambi[2];
         DevBuf mid;  // 7 mid planes by frame % 7
-        DevBuf sums[2], psums[2];
-        HostBuf hsums[2], hpsums[2];
+        DevBuf sums[2], psums[2], fs[2];
+        HostBuf hsums[2], hpsums[2], hfs[2];
         std::vector<std::vector<uint8_t>> masks;
     };
     std::vector<CropBufs> cb;
@@ -192,21 +303,21 @@ private:
     }
 
     // queue (src/main.cpp:516-574): the GPU work of frame f
-    void queue(long f, const void *src, const void *const *encp, hipStream_t st)
+    void queue(long f, const es_frame &src, const es_frame *encp, hipStream_t st)
     {
         const int at = (int)(f % 2);
         for (size_t c = 0; c < crops.size(); c++) {
             if (!crop_active((int)c, f)) continue;
             const PlaneIn p = src_plane(src, (int)c);
-            cb[c].cambi[at]->launch(p.ptr, p.pitch, p.bytes, p.shift, st);
+            cb[c].cambi[at]->launch(p.ptr, p.pitch, p.bytes, p.shift, st, p.rshift);
         }
         std::vector<bool> wrote(crops.size(), false);
         for (size_t e = 0; e < enc.size(); e++) {
             if (!active((int)e, f)) continue;
             const int c = enc[e].crop;
             const Crop &cr = crops[c];
-            const PlaneIn pe = enc_plane(encp[e], (int)e);
-            eb[e].cambi[at]->launch(pe.ptr, pe.pitch, pe.bytes, pe.shift, st);
+            const PlaneIn pe = enc_plane(encp[e]);
+            eb[e].cambi[at]->launch(pe.ptr, pe.pitch, pe.bytes, pe.shift, st, pe.rshift);
             WsPair pr{};
             pr.r = src_plane(src, c);
             pr.d = pe;
@@ -221,6 +332,16 @@ private:
             launch_window_stats(dp, 1, cr.w, cr.h, cfg.win, true, st);
             arena().done(st);
             hip_check(hipMemcpyAsync(eb[e].hsums[at].p, eb[e].sums[at].p, eb[e].sums[at].bytes, hipMemcpyDeviceToHost, st), "reading figures");
+            if (chroma == 1) {
+                // plane_sse x 3 and color_sums (src/main.cpp:543-554) as one job
+                unsigned long long *out = eb[e].fs[at].as<unsigned long long>();
+                launch_clear(out, kFrameSums, nullptr, 0, st);
+                const FrameSumJob j = frame_job(src, encp[e], (int)e, color_encode(c, f) == (int)e, out);
+                const auto *dj = (const FrameSumJob *)arena().stage(&j, sizeof j, st);
+                launch_frame_sums(dj, 1, cr.h, st);
+                arena().done(st);
+                hip_check(hipMemcpyAsync(eb[e].hfs[at].p, out, kFrameSums * 8, hipMemcpyDeviceToHost, st), "reading figures");
+            }
             if (f >= 2 * kRadius) {
                 const long centre = f - kRadius;
                 PersistJob j{};
@@ -261,6 +382,12 @@ private:
                                                 eb[e].masks[f % (kRadius + 1)].data(), ti, to);
             s.quantile.push_back(q);
             acc.add_frame(score, src_score[s.crop], ti, to);
+            if (chroma == 1) {
+                const int ce = color_encode(s.crop, f);
+                fold_chroma((int)e, f, eb[e].hfs[at].as<unsigned long long>(), eb[ce].hfs[at].as<unsigned long long>());
+            } else {
+                no_chroma((int)e);
+            }
             if (f >= 2 * kRadius) {
                 const long centre = f - kRadius;
                 long long tl[3];
@@ -399,6 +526,7 @@ public:
         tot_or.alloc((size_t)B * E * 4 * 8);
         tot_last.alloc((size_t)B * E * 3 * 8);
         quant.alloc((size_t)B * E * 8);
+        fsums.alloc((size_t)B * E * kFrameSums * 8);
         scratch.alloc((size_t)B * E * max_windows * 8);
         {
             const int chunks = (max_windows + 255) / 256;
@@ -413,10 +541,11 @@ public:
         }
         // descriptors: device arrays, staged from pinned host memory per round (two staging buffers)
         desc_bytes = align(P * sizeof(CambiPic)) + align(((size_t)P * kScales + 1) * 4) + align((size_t)B * E * sizeof(WsPair)) +
-                     align((size_t)C * sizeof(LastingJob)) + align((size_t)B * E * sizeof(CollectJob));
+                     align((size_t)C * sizeof(LastingJob)) + align((size_t)B * E * sizeof(CollectJob)) +
+                     align((size_t)B * E * sizeof(FrameSumJob));
         desc.alloc(desc_bytes);
         res_bytes = align((size_t)P * 8) + align((size_t)B * E * 12 * 8) + align((size_t)B * E * 4 * 8) +
-                    align((size_t)B * E * 3 * 8) + align((size_t)B * E * 8);
+                    align((size_t)B * E * 3 * 8) + align((size_t)B * E * 8) + align((size_t)B * E * kFrameSums * 8);
         for (int k = 0; k < 2; k++) {
             stage_buf[k].alloc(desc_bytes);
             res[k].alloc(res_bytes);
@@ -460,8 +589,9 @@ public:
 
     // The rounds run on the scorer's own stream (a graph cannot be captured on the null stream), ordered after the
     // caller's earlier work on `st`, and the caller's later work on `st` waits for them.
-    void submit(int n, const void *const *src, const void *const *encp, hipStream_t st) override
+    void submit(int n, const es_frame *src, const es_frame *encp, hipStream_t st) override
     {
+        check_frames(n, src, encp);
         hip_check(hipEventRecord(joined, st), "event");
         hip_check(hipStreamWaitEvent(work, joined, 0), "event");
         for (int i = 0; i < n; i += B) {
@@ -517,7 +647,7 @@ private:
     DevBuf consts, planes, list, segs, sums, counts, thist, thsum, topk, scores;
     std::vector<CropDev> cd;
     std::vector<EncDev> ed;
-    DevBuf tot_int, tot_or, tot_last, quant, scratch, desc, lout;
+    DevBuf tot_int, tot_or, tot_last, quant, fsums, scratch, desc, lout;
     DevBuf cwork_int, cwork_dbl, cwork_last, cwork_bins, cwork_fhist;
     CollectWork cwork{};
     size_t lout_words = 0;
@@ -542,13 +672,15 @@ private:
     size_t o_ws() const { return o_first() + align(((size_t)P * kScales + 1) * 4); }
     size_t o_last() const { return o_ws() + align((size_t)B * enc.size() * sizeof(WsPair)); }
     size_t o_col() const { return o_last() + align((size_t)crops.size() * sizeof(LastingJob)); }
+    size_t o_fs() const { return o_col() + align((size_t)B * enc.size() * sizeof(CollectJob)); }
     size_t r_scores() const { return 0; }
     size_t r_ti() const { return align((size_t)P * 8); }
     size_t r_to() const { return r_ti() + align((size_t)B * enc.size() * 12 * 8); }
     size_t r_tl() const { return r_to() + align((size_t)B * enc.size() * 4 * 8); }
     size_t r_q() const { return r_tl() + align((size_t)B * enc.size() * 3 * 8); }
+    size_t r_fs() const { return r_q() + align((size_t)B * enc.size() * 8); }
 
-    void fill(int p, long f0, int n, const void *const *src, const void *const *encp)
+    void fill(int p, long f0, int n, const es_frame *src, const es_frame *encp)
     {
         char *h = stage_buf[p].as<char>();
         std::memset(h, 0, desc_bytes);
@@ -558,6 +690,7 @@ private:
         WsPair *ws = (WsPair *)(h + o_ws());
         LastingJob *lj = (LastingJob *)(h + o_last());
         CollectJob *cj = (CollectJob *)(h + o_col());
+        FrameSumJob *fj = (FrameSumJob *)(h + o_fs());
         int t = 0;
         for (int i = 0; i < B; i++) {
             const long f = f0 + i;
@@ -577,7 +710,7 @@ private:
                     pc.toff[s] = lay[c].toff[s];
                 }
                 if (on) {
-                    pc.in = r < C ? src_plane(src[i], r) : enc_plane(encp[(size_t)i * E + (r - C)], r - C);
+                    pc.in = r < C ? src_plane(src[i], r) : enc_plane(encp[(size_t)i * E + (r - C)]);
                     if (cfg.depth == 8) {
                         pc.in.shift += 2;
                         pc.in.from8 = 1;
@@ -605,7 +738,10 @@ private:
                 CollectJob &j = cj[(size_t)i * E + e];
                 if (!on) continue;
                 w.r = src_plane(src[i], c);
-                w.d = enc_plane(encp[(size_t)i * E + e], e);
+                w.d = enc_plane(encp[(size_t)i * E + e]);
+                if (chroma == 1)
+                    fj[(size_t)i * E + e] = frame_job(src[i], encp[(size_t)i * E + e], e, color_encode(c, f) == e,
+                                                      fsums.as<unsigned long long>() + ((size_t)i * E + e) * kFrameSums);
                 w.w = cr.w;
                 w.h = cr.h;
                 w.up = up;
@@ -677,6 +813,7 @@ private:
             hip_check(hipMemcpyAsync(desc.p, stage_buf[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
             launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
             if (cfg.win > 16) launch_clear(lout.as<unsigned long long>(), (long long)lout_words, nullptr, 0, st);
+            if (chroma == 1) launch_clear(fsums.as<unsigned long long>(), (long long)B * E * kFrameSums, nullptr, 0, st);
             break;
         case 1: launch_cambi_base(pics, nullptr, P, max_base_tiles, st); break;
         case 2:
@@ -691,17 +828,21 @@ private:
             break;
         case 5: launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st); break;
         case 6: launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st); break;
-        case 7: launch_lasting(dptr<LastingJob>(o_last()), C, max_w, max_h, cfg.win, st); break;
-        case 8:
+        case 7:
+            if (chroma == 1) launch_frame_sums(dptr<FrameSumJob>(o_fs()), B * E, max_h, st);
+            break;
+        case 8: launch_lasting(dptr<LastingJob>(o_last()), C, max_w, max_h, cfg.win, st); break;
+        case 9:
             launch_collect(dptr<CollectJob>(o_col()), B, E, cwork, st);
             break;
-        case 9: {
+        case 10: {
             char *r = res[p].as<char>();
             hip_check(hipMemcpyAsync(r + r_scores(), scores.p, (size_t)P * 8, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_ti(), tot_int.p, tot_int.bytes, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_to(), tot_or.p, tot_or.bytes, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_tl(), tot_last.p, tot_last.bytes, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_q(), quant.p, quant.bytes, hipMemcpyDeviceToHost, st), "results");
+            if (chroma == 1) hip_check(hipMemcpyAsync(r + r_fs(), fsums.p, fsums.bytes, hipMemcpyDeviceToHost, st), "results");
             break;
         }
         }
@@ -734,14 +875,15 @@ private:
             return;
         }
         for (int k = 0; k < ES_STAGES; k++) {
-            if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, st, &sgraph[p][k]);
             hip_check(hipEventRecord(ev[p][k], st), "stage event");
+            if (k == 7 && chroma != 1) continue;  // no work in the stage: nothing to capture
+            if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, st, &sgraph[p][k]);
             hip_check(hipGraphLaunch(sexec[p][k], st), "graph launch");
         }
         hip_check(hipEventRecord(ev[p][ES_STAGES], st), "stage event");
     }
 
-    void round(int n, const void *const *src, const void *const *encp, hipStream_t st)
+    void round(int n, const es_frame *src, const es_frame *encp, hipStream_t st)
     {
         const int p = (int)(rounds % 2);
         if (pend[p].n) fold(p);
@@ -771,6 +913,7 @@ private:
         const double *to = (const double *)(r + r_to());
         const long long *tl = (const long long *)(r + r_tl());
         const double *q = (const double *)(r + r_q());
+        const unsigned long long *fs = (const unsigned long long *)(r + r_fs());
         const int C = (int)crops.size(), E = (int)enc.size();
         for (int i = 0; i < pend[p].n; i++) {
             const long f = pend[p].f0 + i;
@@ -783,6 +926,10 @@ private:
                 s.cambi_src.push_back(src);
                 s.quantile.push_back(q[k]);
                 segment(e, f).add_frame(score, src, ti + k * 12, to + k * 4);
+                if (chroma == 1)
+                    fold_chroma(e, f, fs + k * kFrameSums, fs + ((size_t)i * E + color_encode(s.crop, f)) * kFrameSums);
+                else
+                    no_chroma(e);
                 if (f >= 2 * kRadius) segment(e, f - kRadius).add_lasting(tl + k * 3);
             }
         }
@@ -819,6 +966,17 @@ int es_scorer_destroy(es_scorer *s)
 }
 
 int es_scorer_submit(es_scorer *s, int nframes, const void *const *src, const void *const *enc, es_stream stream)
+{
+    return guard([&] {
+        require(s && src && enc && nframes >= 0, "bad argument");
+        ensure_init();
+        std::vector<es_frame> fs, fe;
+        s->impl->luma_frames(nframes, src, enc, fs, fe);
+        s->impl->submit(nframes, fs.data(), fe.data(), hs(stream));
+    });
+}
+
+int es_scorer_submit_frames(es_scorer *s, int nframes, const es_frame *src, const es_frame *enc, es_stream stream)
 {
     return guard([&] {
         require(s && src && enc && nframes >= 0, "bad argument");
@@ -862,14 +1020,9 @@ int es_scorer_segment(es_scorer *s, int encode, int seg, es_segment *out, uint64
         require(seg < (int)segs.size(), "no such segment");
         Accum a;
         if (seg >= 0) {
-            a = segs[seg];
-            if (dynamic_cast<BatchedScorer *>(&b)) b.segment_hist(encode, seg, a);
+            a = b.seg_accum(encode, seg);
         } else {
-            for (size_t k = 0; k < segs.size(); k++) {
-                Accum x = segs[k];
-                if (dynamic_cast<BatchedScorer *>(&b)) b.segment_hist(encode, (int)k, x);
-                a.merge(x);
-            }
+            for (size_t k = 0; k < segs.size(); k++) a.merge(b.seg_accum(encode, (int)k));
         }
         a.fill(out);
         if (hist) std::memcpy(hist, a.hist.data(), a.hist.size() * 8);
@@ -886,6 +1039,95 @@ int es_scorer_cambi(es_scorer *s, int encode, long n, double *cambi, double *cam
         require(n >= 0 && n <= (long)e.cambi.size(), "more frames than scored");
         if (cambi) std::copy(e.cambi.begin(), e.cambi.begin() + n, cambi);
         if (cambi_source) std::copy(e.cambi_src.begin(), e.cambi_src.begin() + n, cambi_source);
+    });
+}
+
+int es_scorer_psnr(es_scorer *s, int encode, long n, double *psnr)
+{
+    return guard([&] {
+        require(s && psnr, "null argument");
+        ScorerBase &b = *s->impl;
+        require(encode >= 0 && encode < (int)b.enc.size(), "no such encode");
+        const EncodeState &e = b.enc[encode];
+        require(n >= 0 && 3 * (size_t)n <= e.psnr.size(), "more frames than scored");
+        std::copy(e.psnr.begin(), e.psnr.begin() + 3 * n, psnr);
+    });
+}
+
+int es_scorer_set_vmaf(es_scorer *s, int encode, long n, const double *vmaf)
+{
+    return guard([&] {
+        require(s && (vmaf || n == 0), "null argument");
+        ScorerBase &b = *s->impl;
+        require(encode >= 0 && encode < (int)b.enc.size(), "no such encode");
+        EncodeState &e = b.enc[encode];
+        require(n >= 0 && n <= (long)e.cambi.size(), "more frames than scored");
+        e.vmaf.assign(vmaf, vmaf + n);
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+// The report's figures of every encode: the segments that have frames, their merge and the per-frame lists.
+std::vector<ReportEncode> report_of(ScorerBase &b)
+{
+    std::vector<ReportEncode> out(b.enc.size());
+    for (size_t e = 0; e < b.enc.size(); e++) {
+        ReportEncode &r = out[e];
+        Accum whole;
+        for (int k = 0; k < b.scored_segments((int)e); k++) {
+            const Accum a = b.seg_accum((int)e, k);
+            whole.merge(a);
+            es_segment sg;
+            a.fill(&sg);
+            r.segments.push_back(sg);
+        }
+        whole.fill(&r.whole);
+        const EncodeState &st = b.enc[e];
+        r.vmaf = st.vmaf;
+        r.cambi = st.cambi;
+        r.cambi_source = st.cambi_src;
+        for (size_t i = 0; i < st.psnr.size(); i += 3) r.psnr_y.push_back(st.psnr[i]);
+    }
+    return out;
+}
+
+void check_info(const es_run_info *info)
+{
+    require(info != nullptr, "null run description");
+    require(info->fps > 0, "frame rate must be positive");
+}
+
+}  // namespace
+
+extern "C" {
+
+int es_scorer_write_json(es_scorer *s, const es_run_info *info, const char *path)
+{
+    return guard([&] {
+        require(s && path, "null argument");
+        check_info(info);
+        const std::vector<ReportEncode> r = report_of(*s->impl);
+        FILE *f = std::fopen(path, "w");
+        require(f != nullptr, std::string("cannot write ") + path);
+        write_json(f, *info, s->impl->frames, r);
+        require(std::fclose(f) == 0, std::string("cannot write ") + path);
+    });
+}
+
+int es_scorer_write_summary(es_scorer *s, const es_run_info *info, double seconds, const char *path)
+{
+    return guard([&] {
+        require(s != nullptr, "null scorer");
+        check_info(info);
+        const std::vector<ReportEncode> r = report_of(*s->impl);
+        FILE *f = path ? std::fopen(path, "w") : stdout;
+        require(f != nullptr, std::string("cannot write ") + (path ? path : ""));
+        write_summary(f, *info, seconds, s->impl->frames, r);
+        if (path) require(std::fclose(f) == 0, std::string("cannot write ") + path);
+        else std::fflush(stdout);
     });
 }
 
@@ -939,14 +1181,13 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             BatchedScorer sc(c);
             hipStream_t st = nullptr;
             hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "creating a stream");
-            std::vector<const void *> e((size_t)need * ne);
-            for (int f = 0; f < need; f++)
-                for (int k = 0; k < ne; k++) e[(size_t)f * ne + k] = enc[(size_t)f * ne + k];
-            sc.submit(t.batch, src, e.data(), st);
+            std::vector<es_frame> fs, fe;
+            sc.luma_frames(need, src, enc, fs, fe);
+            sc.submit(t.batch, fs.data(), fe.data(), st);
             sc.sync();
             hip_check(hipStreamSynchronize(st), "tuning");
             const auto t0 = std::chrono::steady_clock::now();
-            sc.submit(2 * t.batch, src + t.batch, e.data() + (size_t)t.batch * ne, st);
+            sc.submit(2 * t.batch, fs.data() + t.batch, fe.data() + (size_t)t.batch * ne, st);
             sc.sync();
             hip_check(hipStreamSynchronize(st), "tuning");
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -37,6 +37,11 @@ struct Accum {
     double sh_all_r[2] = {}, sh_all_x[2] = {}, sh_st_r[2] = {}, sh_st_x[2] = {};
     double own = 0, mine = 0;
     std::vector<uint64_t> hist = std::vector<uint64_t>((size_t)6 * kHistBins, 0);  // amount x3, matching x2, lasting
+    // beside the measures (src/stats.hpp:37-39): PSNR and mean colour sums of the frames with chroma, and VMAF
+    double psnr[3] = {}, color_r[3] = {}, color_d[3] = {};
+    uint64_t cframes = 0;
+    double vmaf = 0, vmaf_min = 1e9;
+    uint64_t vframes = 0;
 
     void add(int k, double pct) { hist[(size_t)k * kHistBins + hist_bin_host(pct)]++; }
     uint64_t count(int k) const
@@ -66,6 +71,31 @@ struct Accum {
             sh_st_x[i] += o.sh_st_x[i];
         }
         for (size_t b = 0; b < hist.size(); b++) hist[b] += o.hist[b];
+        for (int i = 0; i < 3; i++) {
+            psnr[i] += o.psnr[i];
+            color_r[i] += o.color_r[i];
+            color_d[i] += o.color_d[i];
+        }
+        cframes += o.cframes;
+        vmaf += o.vmaf;
+        vmaf_min = std::min(vmaf_min, o.vmaf_min);
+        vframes += o.vframes;
+    }
+    // A frame's PSNR (y, cb, cr) and the mean colours of the source and the encode (src/main.cpp:594, :601-605).
+    void add_chroma(const double *p, const double *cr, const double *cd)
+    {
+        for (int i = 0; i < 3; i++) {
+            psnr[i] += p[i];
+            color_r[i] += cr[i];
+            color_d[i] += cd[i];
+        }
+        cframes++;
+    }
+    void add_vmaf(double v)
+    {
+        vmaf += v;
+        vmaf_min = std::min(vmaf_min, v);
+        vframes++;
     }
     // A frame's exact totals ([all 6, structured 6] window sums, oriented totals all_r, all_d, st_r, st_d).
     void add_frame(double score, double score_src, const long long *ti, const double *to)
@@ -138,15 +168,25 @@ inline void Accum::fill(es_segment *o) const
     o->n_lasting = n[5];
     o->median_lasting = med[5];
     o->tenth_lasting = ten[5];
-    // Accum::score (src/stats.hpp:70-82) without fidelity
+    for (int i = 0; i < 3; i++) {
+        o->psnr[i] = psnr[i];
+        o->color_source[i] = color_r[i];
+        o->color_encode[i] = color_d[i];
+    }
+    o->chroma_frames = cframes;
+    o->vmaf = vmaf;
+    o->vmaf_min = vmaf_min;
+    o->vmaf_frames = vframes;
+    // Accum::score (src/stats.hpp:70-82); fidelity only when every frame has its VMAF
+    o->fidelity = frames && vframes == frames ? clamp100(vmaf / (double)frames) : NAN;
     o->texture = n[5] >= 100 ? clamp100(med[5]) / 2 + clamp100(ten[5]) / 2 : NAN;
     o->banding = frames ? clamp100(100.0 * (1 - (cambi - cambi_source) / (double)frames / 5.0)) : NAN;
     const double kept = all_r[0] > 0 ? 100.0 * std::sqrt(all_d[0] / all_r[0]) : NAN;
     o->grain = clamp100(100.0 - std::fabs(kept - 100.0));
     if (std::isnan(kept)) o->grain = NAN;
-    const double v[3] = {o->texture, o->banding, o->grain}, w[3] = {0.30, 0.25, 0.15};
+    const double v[4] = {o->fidelity, o->texture, o->banding, o->grain}, w[4] = {0.30, 0.30, 0.25, 0.15};
     double total = 0, weight = 0;
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < 4; i++)
         if (!std::isnan(v[i])) {
             total += v[i] * w[i];
             weight += w[i];

@@ -64,7 +64,10 @@ class Segment(C.Structure):
                 ("median_amount", C.c_double * 3), ("tenth_amount", C.c_double * 3),
                 ("median_matching", C.c_double * 2), ("tenth_matching", C.c_double * 2),
                 ("median_lasting", C.c_double), ("tenth_lasting", C.c_double),
-                ("texture", C.c_double), ("banding", C.c_double), ("grain", C.c_double), ("score", C.c_double)]
+                ("texture", C.c_double), ("banding", C.c_double), ("grain", C.c_double), ("score", C.c_double),
+                ("psnr", C.c_double * 3), ("color_source", C.c_double * 3), ("color_encode", C.c_double * 3),
+                ("chroma_frames", _u64), ("vmaf", C.c_double), ("vmaf_min", C.c_double), ("vmaf_frames", _u64),
+                ("fidelity", C.c_double)]
 
     def as_dict(self):
         out = {}
@@ -72,6 +75,22 @@ class Segment(C.Structure):
             v = getattr(self, name)
             out[name] = list(v) if hasattr(v, "__len__") else v
         return out
+
+
+ES_FMT_YUV420P, ES_FMT_YUV420P10, ES_FMT_NV12, ES_FMT_P010 = 0, 1, 2, 3
+FORMATS = {"yuv420p": ES_FMT_YUV420P, "yuv420p10": ES_FMT_YUV420P10, "nv12": ES_FMT_NV12, "p010": ES_FMT_P010}
+
+
+class Frame(C.Structure):
+    """es_frame: a decoded 4:2:0 frame in device memory; pitches in bytes."""
+    _fields_ = [("y", _vp), ("u", _vp), ("v", _vp), ("ypitch", _i), ("cpitch", _i), ("semi", _i), ("bytes", _i),
+                ("shift", _i), ("depth", _i)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("source", C.c_char_p), ("paths", C.POINTER(C.c_char_p)), ("model", C.c_char_p), ("start", C.c_double),
+                ("offset", C.c_double), ("fps", C.c_double), ("segment_seconds", C.c_double), ("width", _i),
+                ("height", _i)]
 
 
 class SynthDesc(C.Structure):
@@ -117,6 +136,17 @@ for _n, _a in [
     ("es_scorer_cambi", [_vp, _i, C.c_long, _vp, _vp]), ("es_scorer_thresholds", [_vp, _i, C.c_long, _vp]),
     ("es_scorer_stage_ms", [_vp, _vp, C.POINTER(C.c_long)]), ("es_copy_kernel", [_vp, _vp, _sz, _vp]),
     ("es_synth_frame", [C.POINTER(SynthDesc), _i, _vp, _i, _vp]),
+    ("es_frame_view", [_i, _vp, _vp, _vp, _i, _i, C.POINTER(Frame)]),
+    ("es_frame_planes", [C.POINTER(Frame), _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp]),
+    ("es_plane_sse", [_vp, _vp, _vp, _vp, _sz, _vp, _vp]),
+    ("es_color_sums", [_vp, _vp, _sz, _i, _vp, _vp]),
+    ("es_color_values", [_vp, _sz, _vp]),
+    ("es_psnr", [_u64, _sz, _i, _dp]),
+    ("es_scorer_submit_frames", [_vp, _i, C.POINTER(Frame), C.POINTER(Frame), _vp]),
+    ("es_scorer_psnr", [_vp, _i, C.c_long, _vp]),
+    ("es_scorer_set_vmaf", [_vp, _i, C.c_long, _vp]),
+    ("es_scorer_write_json", [_vp, C.POINTER(RunInfo), C.c_char_p]),
+    ("es_scorer_write_summary", [_vp, C.POINTER(RunInfo), C.c_double, C.c_char_p]),
     ("es_synth_frame_host", [C.POINTER(SynthDesc), _i, _vp, _i, _i]),
 ]:
     _sig(_n, _a)
@@ -336,6 +366,39 @@ class Scorer:
         self.ptr = p.value
         self.n_encodes = len(encodes)
 
+    def submit_frames(self, src_frames, enc_frames, stream=None):
+        """src_frames: n Frame structures; enc_frames: n rows of n_encodes Frame structures."""
+        n = len(src_frames)
+        src = (Frame * n)(*src_frames)
+        flat = [f for row in enc_frames for f in row]
+        enc = (Frame * max(1, len(flat)))(*flat)
+        check(lib.es_scorer_submit_frames(self.ptr, n, src, enc, _ptr(stream)))
+
+    def psnr(self, e, n):
+        out = np.empty((n, 3))
+        check(lib.es_scorer_psnr(self.ptr, e, n, out.ctypes.data))
+        return out
+
+    def set_vmaf(self, e, vmaf):
+        v = np.ascontiguousarray(vmaf, np.float64)
+        check(lib.es_scorer_set_vmaf(self.ptr, e, v.size, v.ctypes.data))
+
+    def _info(self, info):
+        paths = [p.encode() for p in info["paths"]]
+        self._paths = (C.c_char_p * max(1, len(paths)))(*paths)
+        return RunInfo(info["source"].encode(), self._paths, info.get("model", "vmaf_v0.6.1").encode(),
+                       info.get("start", 0.0), info.get("offset", 0.0), info["fps"], info["segment_seconds"],
+                       info["width"], info["height"])
+
+    def write_json(self, info, path):
+        """info: dict of source, paths, model, start, offset, fps, segment_seconds, width, height."""
+        ri = self._info(info)
+        check(lib.es_scorer_write_json(self.ptr, C.byref(ri), str(path).encode()))
+
+    def write_summary(self, info, seconds, path=None):
+        ri = self._info(info)
+        check(lib.es_scorer_write_summary(self.ptr, C.byref(ri), seconds, None if path is None else str(path).encode()))
+
     def submit(self, src_ptrs, enc_ptrs, stream=None):
         """src_ptrs: device pointers of n source frames; enc_ptrs: n x n_encodes device pointers."""
         n = len(src_ptrs)
@@ -369,7 +432,7 @@ class Scorer:
         return a, b
 
     STAGES = ("upload", "cambi_base", "cambi_scale", "cambi_cvalues", "cambi_topk", "cambi_finish", "window_stats",
-              "persist", "collect", "results")
+              "frame_sums", "persist", "collect", "results")
 
     def stage_ms(self):
         ms = np.zeros(len(self.STAGES))
@@ -387,6 +450,43 @@ class Scorer:
             lib.es_scorer_destroy(self.ptr)
         except Exception:
             pass
+
+
+def frame_view(fmt, y, u, v, ypitch, cpitch):
+    """es_frame_view: the Frame of device planes y, u (and v when planar) of `fmt` (a name of FORMATS or its code)."""
+    f = Frame()
+    check(lib.es_frame_view(FORMATS.get(fmt, fmt), _ptr(y), _ptr(u), _ptr(v), ypitch, cpitch, C.byref(f)))
+    return f
+
+
+def frame_planes(frame, x0, y0, pd, y16, u, v, w, h, stream=None):
+    check(lib.es_frame_planes(C.byref(frame), x0, y0, pd, _ptr(y16), _ptr(u), _ptr(v), w, h, _ptr(stream)))
+
+
+def plane_sse(a, b, n, sse, floats=False, stream=None):
+    """Adds the squared error of n samples of device planes a and b (uint16, or float32 with floats) to the device
+    uint64 *sse."""
+    if floats:
+        check(lib.es_plane_sse(None, None, _ptr(a), _ptr(b), n, _ptr(sse), _ptr(stream)))
+    else:
+        check(lib.es_plane_sse(_ptr(a), _ptr(b), None, None, n, _ptr(sse), _ptr(stream)))
+
+
+def color_sums(u, v, n, up, sums, stream=None):
+    check(lib.es_color_sums(_ptr(u), _ptr(v), n, up, _ptr(sums), _ptr(stream)))
+
+
+def color_values(sums, n):
+    s = np.ascontiguousarray(sums, np.uint64)
+    out = np.empty(3)
+    check(lib.es_color_values(s.ctypes.data, n, out.ctypes.data))
+    return out
+
+
+def psnr(sse, n, depth):
+    out = C.c_double()
+    check(lib.es_psnr(int(sse), int(n), depth, C.byref(out)))
+    return out.value
 
 
 def copy_kernel(dst, src, nbytes, stream=None):

@@ -131,6 +131,44 @@ int es_select_rank_u64(const uint64_t *keys, size_t n, size_t rank, uint64_t *ou
  * (NaN for an empty histogram) and host counts[h] (may be NULL). */
 int es_hist_quantiles(const uint64_t *hists, int nhist, const double *qs, int nq, double *out, uint64_t *counts);
 
+/* ---- frame ingest and the PSNR and colour reductions (src/kernels.cu:280-310, :453-484; src/main.cpp:128-153,
+ *      :273-277, :496-505, :543-556) ---- */
+
+/* Decoded 4:2:0 pixel formats at 8 or 10 bits (the table of view_of, src/main.cpp:135-147). */
+#define ES_FMT_YUV420P 0   /* planar, 1 byte per sample */
+#define ES_FMT_YUV420P10 1 /* planar, 2 bytes per sample, the value in the low 10 bits */
+#define ES_FMT_NV12 2      /* a luma plane and one plane of interleaved u, v; 1 byte per sample */
+#define ES_FMT_P010 3      /* as NV12 with 2 bytes per sample, the value in the high 10 bits */
+
+/* A decoded frame as the kernels read it (struct Frame, src/kernels.cu:284-287): planar or semi-planar (u and v
+ * interleaved in plane u; v unused) 4:2:0; a sample's value is the stored word shifted right by `shift` and has
+ * `depth` bits.  Pitches are in bytes. */
+typedef struct {
+    const void *y, *u, *v;
+    int ypitch, cpitch, semi, bytes, shift, depth;
+} es_frame;
+/* The description of a frame of `format` (view_of, src/main.cpp:128-153): ES_EINVAL for an unsupported format. */
+int es_frame_view(int format, const void *y, const void *u, const void *v, int ypitch, int cpitch, es_frame *out);
+/* frame_planes (src/kernels.cu:297-310): the frame cropped at (x0, y0) to w x h (both even) as packed luma y16
+ * (w * h, 4-byte aligned) and chroma u, v ((w / 2) * (h / 2)), brought to the pipeline depth pd (8 or 10; an 8-bit
+ * frame in a 10-bit pipeline is shifted up by 2, as libvmaf does).  Queued on s. */
+int es_frame_planes(const es_frame *f, int x0, int y0, int pd, uint16_t *y16, uint16_t *u, uint16_t *v, int w, int h,
+                    es_stream s);
+/* plane_sse (src/kernels.cu:454-465): adds to the device *sse the sum of squared differences of n samples of a and b,
+ * or, when a is NULL, of the floats fa and fb truncated to integers.  Integer adds: the same total on every run. */
+int es_plane_sse(const uint16_t *a, const uint16_t *b, const float *fa, const float *fb, size_t n, uint64_t *sse,
+                 es_stream s);
+/* color_sums (src/kernels.cu:470-484) in exact integers: adds to the device sums[0 .. 2] the sums over n samples of
+ * u << up, v << up, and 2^23 times the colour strength sqrtf(du^2 + dv^2) (du = (u << up) - 512, likewise dv) rounded
+ * to single precision as the reference's is.  Samples must be at most 10-bit after the shift. */
+int es_color_sums(const uint16_t *u, const uint16_t *v, size_t n, int up, uint64_t *sums, es_stream s);
+/* The mean colour of n samples from es_color_sums's host copy: out[0 .. 2] = mean u, mean v, mean strength (each
+ * frame's contribution to Accum's color, src/main.cpp:603-604). */
+int es_color_values(const uint64_t *sums, size_t n, double *out);
+/* psnr (src/main.cpp:273-277): libvmaf's PSNR of a squared error over n samples at bit depth `depth`, peak
+ * 2^depth - 1, capped at 6 depth + 12 dB. */
+int es_psnr(uint64_t sse, size_t n, int depth, double *out);
+
 /* ---- the whole measure path of one source against its encodes (src/main.cpp:516-643 queue/collect,
  *      src/stats.hpp Accum) ---- */
 
@@ -162,8 +200,11 @@ typedef struct {
     int cv_waves;     /* batched form: persistent c-value waves per CU (0 = 16) */
 } es_scorer_config;
 
-/* One segment of one encode (Accum, src/stats.hpp:36-82, measure fields), with its histograms' quantiles and the
- * score parts (fidelity, from VMAF, is outside this path: left out of the weighted mean as an unmeasured part is). */
+/* One segment of one encode (Accum, src/stats.hpp:36-82), with its histograms' quantiles and the score parts.
+ * Fidelity is VMAF's, which this path does not compute: with VMAF attached to every frame of the segment
+ * (es_scorer_set_vmaf) it is clamp(vmaf / frames) as in the reference, otherwise NaN and left out of the weighted
+ * mean as a part that could not be measured is.  The PSNR and colour fields are sums over the frames, as Accum keeps
+ * them; they stay 0 for frames submitted without chroma planes (chroma_frames counts those with). */
 typedef struct {
     uint64_t frames;
     double cambi, cambi_source;
@@ -173,6 +214,12 @@ typedef struct {
     uint64_t n_amount[3], n_matching[2], n_lasting;
     double median_amount[3], tenth_amount[3], median_matching[2], tenth_matching[2], median_lasting, tenth_lasting;
     double texture, banding, grain, score;
+    double psnr[3];                           /* y, cb, cr */
+    double color_source[3], color_encode[3];  /* mean u, mean v, mean strength, at the 10-bit scale */
+    uint64_t chroma_frames;
+    double vmaf, vmaf_min;                    /* sum and minimum (1e9 when none) of the attached VMAF */
+    uint64_t vmaf_frames;
+    double fidelity;
 } es_segment;
 
 int es_scorer_create(const es_scorer_config *cfg, es_scorer **out);
@@ -192,6 +239,12 @@ int es_scorer_destroy(es_scorer *s);
 /* Queues `nframes` frame steps: src[i] is the source's device plane of step i (the full picture), enc[i * n_encodes
  * + e] encode e's (ignored once the encode has ended). Device planes must stay unchanged until es_scorer_sync. */
 int es_scorer_submit(es_scorer *s, int nframes, const void *const *src, const void *const *enc, es_stream stream);
+/* Queues `nframes` frame steps of decoded frames, as es_scorer_submit: src[i] the source's frame of step i (the full
+ * picture), enc[i * n_encodes + e] encode e's.  Each frame carries its own format and pitches (the config's pitch,
+ * bytes and shift are not used) and is read in place.  With chroma planes (u not NULL) the frame step also measures
+ * the PSNR of luma and both chroma planes and the mean colour of both pictures (src/main.cpp:525, :543-556,
+ * :594-605); every frame of a run must carry chroma, or none.  Crops and picture sizes must be even. */
+int es_scorer_submit_frames(es_scorer *s, int nframes, const es_frame *src, const es_frame *enc, es_stream stream);
 /* Waits for the queued work and folds its figures into the segments. */
 int es_scorer_sync(es_scorer *s);
 int es_scorer_frames(es_scorer *s, long *frames);
@@ -201,10 +254,31 @@ int es_scorer_segments(es_scorer *s, int encode, int *count);
 int es_scorer_segment(es_scorer *s, int encode, int seg, es_segment *out, uint64_t *hist);
 /* Per-frame CAMBI of the encode and of the source cropped to it, for frames [0, n). */
 int es_scorer_cambi(es_scorer *s, int encode, long n, double *cambi, double *cambi_source);
+/* Per-frame PSNR of the encode for frames [0, n): psnr[3 i + k] for y, cb, cr (NaN for frames without chroma). */
+int es_scorer_psnr(es_scorer *s, int encode, long n, double *psnr);
+/* Attaches VMAF scores of the encode's frames [0, n) computed elsewhere (libvmaf in the reference, src/main.cpp:698-704):
+ * they fill the segments' vmaf figures and the fidelity part of the score. */
+int es_scorer_set_vmaf(es_scorer *s, int encode, long n, const double *vmaf);
+
+/* What the run's report names besides the figures (src/main.cpp:693-695). */
+typedef struct {
+    const char *source;            /* the source's path */
+    const char *const *paths;      /* each encode's path, n_encodes of them */
+    const char *model;             /* the VMAF model's name */
+    double start, offset, fps, segment_seconds;
+    int width, height;             /* the scored picture */
+} es_run_info;
+/* The run's JSON report in the reference's format (src/main.cpp:693-777: the header, per encode the whole run, the
+ * segments and the per-frame lists; the same field names and digits, null for a figure that could not be measured)
+ * written to `path`. */
+int es_scorer_write_json(es_scorer *s, const es_run_info *info, const char *path);
+/* The run's text summary as the reference prints it (src/main.cpp:695-727): the whole run and the 5 worst segments of
+ * each encode, after a header with the wall time `seconds`; to `path`, or to standard output when path is NULL. */
+int es_scorer_write_summary(es_scorer *s, const es_run_info *info, double seconds, const char *path);
 /* Batched form with profile set: the device milliseconds of each stage summed over the synced rounds, for
  * ES_STAGES stages in the order ES_STAGE_NAMES lists, and the number of rounds. */
-#define ES_STAGES 10
-#define ES_STAGE_NAMES "upload,cambi_base,cambi_scale,cambi_cvalues,cambi_topk,cambi_finish,window_stats,persist,collect,results"
+#define ES_STAGES 11
+#define ES_STAGE_NAMES "upload,cambi_base,cambi_scale,cambi_cvalues,cambi_topk,cambi_finish,window_stats,frame_sums,persist,collect,results"
 int es_scorer_stage_ms(es_scorer *s, double *ms, long *rounds);
 /* Per-frame diagnostics of the last synced frames: the 12 window sums of (encode, frame) when still held, the
  * structure threshold of each frame (NaN when not kept). */

@@ -656,7 +656,11 @@ void orc_hist_merge(orc_hist *h, const orc_hist *o)
     h->n += o->n;
 }
 
-void orc_accum_clear(orc_accum *a) { memset(a, 0, sizeof *a); }
+void orc_accum_clear(orc_accum *a)
+{
+    memset(a, 0, sizeof *a);
+    a->vmaf_min = 1e9; /* src/stats.hpp:38 */
+}
 
 /* Accum::merge (src/stats.hpp:50-64), measure fields. */
 void orc_accum_merge(orc_accum *a, const orc_accum *o)
@@ -681,6 +685,13 @@ void orc_accum_merge(orc_accum *a, const orc_accum *o)
         orc_hist_merge(&a->matching[i], &o->matching[i]);
     }
     orc_hist_merge(&a->lasting, &o->lasting);
+    for (int i = 0; i < 3; i++) {
+        a->psnr[i] += o->psnr[i];
+        a->color_r[i] += o->color_r[i];
+        a->color_d[i] += o->color_d[i];
+    }
+    a->vmaf += o->vmaf;
+    a->vmaf_min = o->vmaf_min < a->vmaf_min ? o->vmaf_min : a->vmaf_min;
 }
 
 static double clamp100(double v) { return v < 0.0 ? 0.0 : v > 100.0 ? 100.0 : v; }
@@ -782,6 +793,94 @@ void orc_collect_lasting(orc_accum *acc, const long long *psums, const uint8_t *
 }
 
 /* ------------------------------------------------------------------------------------------------------------------
+ * Frame ingest, PSNR and colour.
+ * ------------------------------------------------------------------------------------------------------------------ */
+
+/* sample() of src/kernels.cu:289-293: the stored word shifted right by `shift`. */
+static unsigned frame_sample(const void *plane, int pitch, int x, int y, const orc_frame *f)
+{
+    const unsigned char *row = (const unsigned char *)plane + (size_t)y * pitch;
+    return (f->bytes == 1 ? row[x] : ((const uint16_t *)row)[x]) >> f->shift;
+}
+
+/* frame_planes (src/kernels.cu:297-310), pixel by pixel: luma at (x, y), and for even x and y the chroma sample at
+ * ((x + x0) / 2, (y + y0) / 2), u and v interleaved in plane u for a semi-planar frame; all shifted up by pd - depth. */
+void orc_frame_planes(const orc_frame *f, int x0, int y0, int pd, uint16_t *y16, uint16_t *u, uint16_t *v, int w, int h)
+{
+    const int up = pd - f->depth;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            y16[(size_t)y * w + x] = (uint16_t)(frame_sample(f->y, f->ypitch, x + x0, y + y0, f) << up);
+            if ((x & 1) || (y & 1)) continue;
+            const int cx = (x + x0) / 2, cy = (y + y0) / 2;
+            const size_t ci = (size_t)(y / 2) * (w / 2) + x / 2;
+            const unsigned cu = f->semi ? frame_sample(f->u, f->cpitch, 2 * cx, cy, f) : frame_sample(f->u, f->cpitch, cx, cy, f);
+            const unsigned cv = f->semi ? frame_sample(f->u, f->cpitch, 2 * cx + 1, cy, f) : frame_sample(f->v, f->cpitch, cx, cy, f);
+            u[ci] = (uint16_t)(cu << up);
+            v[ci] = (uint16_t)(cv << up);
+        }
+}
+
+/* plane_sse (src/kernels.cu:454-465): the exact sum of squared differences. */
+unsigned long long orc_plane_sse(const uint16_t *a, const uint16_t *b, long long n)
+{
+    unsigned long long s = 0;
+    for (long long i = 0; i < n; i++) {
+        const long long d = (long long)a[i] - b[i];
+        s += (unsigned long long)(d * d);
+    }
+    return s;
+}
+
+/* plane_sse's float form: the differences of the values truncated to integers. */
+unsigned long long orc_plane_sse_f(const float *a, const float *b, long long n)
+{
+    unsigned long long s = 0;
+    for (long long i = 0; i < n; i++) {
+        const long long d = (long long)a[i] - (long long)b[i];
+        s += (unsigned long long)(d * d);
+    }
+    return s;
+}
+
+/* 2^23 sqrtf(s) (IEEE single precision, round to nearest) as an exact integer: a float of at least 1 is a multiple of
+ * 2^-23. */
+unsigned long long orc_strength_fixed(unsigned s)
+{
+    if (s == 0) return 0;
+    const float f = sqrtf((float)s);
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    return (unsigned long long)((b & 0x7FFFFFu) | 0x800000u) << ((b >> 23) - 127);
+}
+
+/* color_sums (src/kernels.cu:470-484): sums of u << up, v << up and of the colour strength sqrtf(du^2 + dv^2) about
+ * the neutral 512.  Canonical form: the strengths, each rounded to single precision as the reference's sqrtf does,
+ * are added exactly (units of 2^-23) instead of in the reference's per-thread float partial sums, whose order
+ * depends on its launch shape. */
+void orc_color_sums(const uint16_t *u, const uint16_t *v, long long n, int up, unsigned long long *sums)
+{
+    unsigned long long su = 0, sv = 0, ss = 0;
+    for (long long i = 0; i < n; i++) {
+        const int uu = u[i] << up, vv = v[i] << up, du = uu - 512, dv = vv - 512;
+        su += (unsigned)uu;
+        sv += (unsigned)vv;
+        ss += orc_strength_fixed((unsigned)(du * du + dv * dv));
+    }
+    sums[0] = su;
+    sums[1] = sv;
+    sums[2] = ss;
+}
+
+/* psnr (src/main.cpp:273-277) */
+double orc_psnr(unsigned long long sse, long long n, int depth)
+{
+    const double mse = (double)sse / (double)n, peak = (double)((1 << depth) - 1);
+    const double v = 10.0 * log10(peak * peak / (mse > 1e-16 ? mse : 1e-16)), cap = 6.0 * depth + 12;
+    return v < cap ? v : cap;
+}
+
+/* ------------------------------------------------------------------------------------------------------------------
  * The whole run of one pair, as src/main.cpp's queue (:516-574) and collect (:576-643) do it frame by frame: rings of
  * 13 source mid planes and 7 encode mid planes and structure masks (src/main.cpp:240, :448, :469, :478), texture
  * from frame 12 on for the centre 6 frames back, into the centre's segment.
@@ -871,6 +970,31 @@ int orc_run_frame(orc_run *r, const uint16_t *src, int src_pitch, const uint16_t
     if (cambi) *cambi = banding;
     if (cambi_source) *cambi_source = banding_src;
     r->frames++;
+    return 0;
+}
+
+int orc_run_frame_yuv(orc_run *r, const uint16_t *const *src, const uint16_t *const *enc, double *cambi,
+                      double *cambi_source, double *psnr)
+{
+    const long f = r->frames;
+    const long long n = (long long)r->c.w * r->c.h, nc = n / 4;
+    const int pd = r->c.depth, up = 10 - pd;
+    const unsigned long long sse[3] = {orc_plane_sse(src[0], enc[0], n), orc_plane_sse(src[1], enc[1], nc),
+                                       orc_plane_sse(src[2], enc[2], nc)};
+    unsigned long long cr[3], cd[3];
+    orc_color_sums(src[1], src[2], nc, up, cr);
+    orc_color_sums(enc[1], enc[2], nc, up, cd);
+    const int rc = orc_run_frame(r, src[0], r->c.w, enc[0], r->c.w, cambi, cambi_source);
+    if (rc) return rc;
+    orc_accum *acc = segment_of(r, f);
+    for (int i = 0; i < 3; i++) {
+        const double p = orc_psnr(sse[i], i ? nc : n, pd);
+        acc->psnr[i] += p;
+        if (psnr) psnr[i] = p;
+        const double sr = i < 2 ? (double)cr[i] : ldexp((double)cr[i], -23), sd = i < 2 ? (double)cd[i] : ldexp((double)cd[i], -23);
+        acc->color_r[i] += sr / (double)nc;
+        acc->color_d[i] += sd / (double)nc;
+    }
     return 0;
 }
 

@@ -98,6 +98,9 @@ typedef struct {
     double sh_all_r[2], sh_all_x[2], sh_st_r[2], sh_st_x[2];
     double own, mine;
     orc_hist amount[3], matching[2], lasting;
+    /* beside the measures: PSNR sums, mean colour sums (source, encode) and VMAF (src/stats.hpp:37-39) */
+    double psnr[3], color_r[3], color_d[3];
+    double vmaf, vmaf_min;
 } orc_accum;
 
 typedef struct {
@@ -110,6 +113,19 @@ void orc_accum_score(const orc_accum *a, double vmaf_total, orc_parts *p);
 void orc_collect_frame(orc_accum *acc, const long long *sums, int windows, uint8_t *mask_out);
 void orc_collect_lasting(orc_accum *acc, const long long *psums, const uint8_t *cmask, int windows);
 
+/* Frame ingest, PSNR and colour (src/kernels.cu:282-310, :453-484; src/main.cpp:273-277, :594-605).  orc_frame has
+ * the layout of struct Frame (src/kernels.cu:284-287); pitches in bytes. */
+typedef struct {
+    const void *y, *u, *v;
+    int ypitch, cpitch, semi, bytes, shift, depth;
+} orc_frame;
+void orc_frame_planes(const orc_frame *f, int x0, int y0, int pd, uint16_t *y16, uint16_t *u, uint16_t *v, int w, int h);
+unsigned long long orc_plane_sse(const uint16_t *a, const uint16_t *b, long long n);
+unsigned long long orc_plane_sse_f(const float *a, const float *b, long long n);
+unsigned long long orc_strength_fixed(unsigned s);
+void orc_color_sums(const uint16_t *u, const uint16_t *v, long long n, int up, unsigned long long *sums);
+double orc_psnr(unsigned long long sse, long long n, int depth);
+
 /* Whole-run driver for one (source, encode) pair, following src/main.cpp:516-643. */
 typedef struct {
     int w, h, depth, up, win, per_segment, window_base, threads;
@@ -119,6 +135,10 @@ orc_run *orc_run_create(const orc_run_config *c);
 void orc_run_destroy(orc_run *r);
 int orc_run_frame(orc_run *r, const uint16_t *src, int src_pitch, const uint16_t *enc, int enc_pitch, double *cambi,
                   double *cambi_source);
+/* The same frame with its chroma: y planes w x h (pitch w), u and v (w / 2) x (h / 2), all at the pipeline depth;
+ * adds the frame's PSNR and both mean colours to its segment (src/main.cpp:594-605). */
+int orc_run_frame_yuv(orc_run *r, const uint16_t *const *src, const uint16_t *const *enc, double *cambi,
+                      double *cambi_source, double *psnr);
 int orc_run_segments(const orc_run *r);
 const orc_accum *orc_run_segment(const orc_run *r, int s);
 

@@ -46,7 +46,13 @@ class Accum(C.Structure):
     _fields_ = [("frames", C.c_uint64), ("cambi", _dp), ("cambi_source", _dp),
                 ("all_r", _dp * 3), ("all_d", _dp * 3), ("st_r", _dp * 3), ("st_d", _dp * 3),
                 ("sh_all_r", _dp * 2), ("sh_all_x", _dp * 2), ("sh_st_r", _dp * 2), ("sh_st_x", _dp * 2),
-                ("own", _dp), ("mine", _dp), ("amount", Hist * 3), ("matching", Hist * 2), ("lasting", Hist)]
+                ("own", _dp), ("mine", _dp), ("amount", Hist * 3), ("matching", Hist * 2), ("lasting", Hist),
+                ("psnr", _dp * 3), ("color_r", _dp * 3), ("color_d", _dp * 3), ("vmaf", _dp), ("vmaf_min", _dp)]
+
+
+class Frame(C.Structure):
+    _fields_ = [("y", _vp), ("u", _vp), ("v", _vp), ("ypitch", _i), ("cpitch", _i), ("semi", _i), ("bytes", _i),
+                ("shift", _i), ("depth", _i)]
 
 
 class Parts(C.Structure):
@@ -99,7 +105,14 @@ _sig("orc_collect_lasting", [C.POINTER(Accum), _vp, _vp, _i])
 _sig("orc_run_create", [C.POINTER(RunConfig)], _vp)
 _sig("orc_run_destroy", [_vp])
 _sig("orc_run_frame", [_vp, _vp, _i, _vp, _i, C.POINTER(_dp), C.POINTER(_dp)], _i)
+_sig("orc_run_frame_yuv", [_vp, _vp, _vp, C.POINTER(_dp), C.POINTER(_dp), _vp], _i)
 _sig("orc_run_segments", [_vp], _i)
+_sig("orc_frame_planes", [C.POINTER(Frame), _i, _i, _i, _vp, _vp, _vp, _i, _i])
+_sig("orc_plane_sse", [_vp, _vp, _ll], C.c_ulonglong)
+_sig("orc_plane_sse_f", [_vp, _vp, _ll], C.c_ulonglong)
+_sig("orc_strength_fixed", [C.c_uint], C.c_ulonglong)
+_sig("orc_color_sums", [_vp, _vp, _ll, _i, _vp])
+_sig("orc_psnr", [C.c_ulonglong, _ll, _i], _dp)
 _sig("orc_run_segment", [_vp, _i], C.POINTER(Accum))
 
 
@@ -228,6 +241,54 @@ def hist_quantile(h, q):
     return lib.orc_hist_quantile(C.byref(h), q)
 
 
+# Formats of decoded frames: (semi-planar, bytes per sample, right shift, depth), as view_of's table
+# (src/main.cpp:135-147).
+FORMATS = {"yuv420p": (0, 1, 0, 8), "yuv420p10": (0, 2, 0, 10), "nv12": (1, 1, 0, 8), "p010": (1, 2, 6, 10)}
+
+
+def frame_planes(fmt, y, u, v, x0, y0, w, h, pd):
+    """frame_planes of a frame held in numpy planes: y (rows x pitch samples); u and v for a planar format, or u the
+    interleaved (rows x 2 * cw) plane of a semi-planar one.  Returns packed y16, u, v."""
+    semi, nbytes, shift, depth = FORMATS[fmt]
+    dt = np.uint8 if nbytes == 1 else np.uint16
+    y = np.ascontiguousarray(y, dt)
+    u = np.ascontiguousarray(u, dt)
+    v = None if semi else np.ascontiguousarray(v, dt)
+    f = Frame(y.ctypes.data, u.ctypes.data, 0 if v is None else v.ctypes.data, y.shape[1] * nbytes,
+              u.shape[1] * nbytes, semi, nbytes, shift, depth)
+    oy = np.empty((h, w), np.uint16)
+    ou = np.empty((h // 2, w // 2), np.uint16)
+    ov = np.empty((h // 2, w // 2), np.uint16)
+    lib.orc_frame_planes(C.byref(f), x0, y0, pd, oy.ctypes.data, ou.ctypes.data, ov.ctypes.data, w, h)
+    return oy, ou, ov
+
+
+def plane_sse(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    if a.dtype == np.float32:
+        return lib.orc_plane_sse_f(a.ctypes.data, b.ctypes.data, a.size)
+    a = a.astype(np.uint16, copy=False)
+    b = b.astype(np.uint16, copy=False)
+    return lib.orc_plane_sse(a.ctypes.data, b.ctypes.data, a.size)
+
+
+def strength_fixed(s):
+    return lib.orc_strength_fixed(int(s))
+
+
+def color_sums(u, v, up):
+    u = np.ascontiguousarray(u, np.uint16)
+    v = np.ascontiguousarray(v, np.uint16)
+    out = np.zeros(3, np.uint64)
+    lib.orc_color_sums(u.ctypes.data, v.ctypes.data, u.size, up, out.ctypes.data)
+    return out
+
+
+def psnr(sse, n, depth):
+    return lib.orc_psnr(int(sse), int(n), depth)
+
+
 class Run:
     """The whole run of one (source, encode) pair (src/main.cpp's queue/collect) on the CPU."""
 
@@ -243,6 +304,16 @@ class Run:
         lib.orc_run_frame(self.ptr, src.ctypes.data, src.shape[1], enc.ctypes.data, enc.shape[1], C.byref(a),
                           C.byref(b))
         return a.value, b.value
+
+    def frame_yuv(self, src, enc):
+        """src, enc: (y, u, v) planes at the pipeline depth.  Returns (cambi, cambi_source, psnr[3])."""
+        keep = [np.ascontiguousarray(p, np.uint16) for p in (*src, *enc)]
+        sp = (_vp * 3)(*[p.ctypes.data for p in keep[:3]])
+        ep = (_vp * 3)(*[p.ctypes.data for p in keep[3:]])
+        a, b = _dp(), _dp()
+        ps = np.zeros(3)
+        assert lib.orc_run_frame_yuv(self.ptr, sp, ep, C.byref(a), C.byref(b), ps.ctypes.data) == 0
+        return a.value, b.value, ps
 
     def segments(self):
         return lib.orc_run_segments(self.ptr)
@@ -279,7 +350,8 @@ def accum_summary(a, vmaf_total=float("nan")):
            "median_matching": [hist_quantile(a.matching[k], 0.5) for k in range(2)],
            "tenth_matching": [hist_quantile(a.matching[k], 0.1) for k in range(2)],
            "median_lasting": hist_quantile(a.lasting, 0.5), "tenth_lasting": hist_quantile(a.lasting, 0.1),
-           "texture": parts.texture, "banding": parts.banding, "grain": parts.grain, "score": parts.score}
+           "texture": parts.texture, "banding": parts.banding, "grain": parts.grain, "score": parts.score,
+           "psnr": list(a.psnr), "color_source": list(a.color_r), "color_encode": list(a.color_d)}
     hists = np.zeros((6, HIST_BINS), np.uint64)
     for k in range(3):
         hists[k] = np.frombuffer(a.amount[k].bins, np.uint64)

@@ -70,3 +70,43 @@ def test_dropin_headers_compile():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         "-I", os.path.join(ROOT, "encode-score_amd", "host"), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_frame_view_follows_the_format_table(es, orc):
+    """es_frame_view's (semi, bytes, shift, depth) per format is view_of's table (src/main.cpp:135-147)."""
+    for name, code in es.FORMATS.items():
+        f = es.frame_view(code, 0x1000, 0x2000, 0x3000, 4096, 2048)
+        assert (f.semi, f.bytes, f.shift, f.depth) == orc.FORMATS[name], name
+        assert (f.ypitch, f.cpitch) == (4096, 2048)
+        assert (f.v is None) == bool(f.semi)
+    with pytest.raises(es.EsError, match="unsupported pixel format"):
+        es.frame_view(7, 0x1000, 0x2000, 0x3000, 64, 64)
+
+
+def test_psnr_and_colour_means_equal_the_oracles(es, orc):
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        n = int(rng.integers(1, 10**7))
+        sse = int(rng.integers(0, 1023 ** 2)) * n // int(rng.integers(1, 1000))
+        for depth in (8, 10):
+            a, b = es.psnr(sse, n, depth), orc.psnr(sse, n, depth)
+            assert a == b
+    assert es.psnr(0, 10, 10) == 72.0
+    sums = np.array([512 * 40, 600 * 40, 5 * 40 << 23], np.uint64)
+    assert list(es.color_values(sums, 40)) == [512.0, 600.0, 5.0]
+
+
+def test_frame_arguments_are_checked_before_any_device_work(es):
+    f = es.frame_view("p010", 0x1000, 0x2000, 0, 4096, 4096)
+    for bad in (dict(w=63), dict(h=0), dict(pd=9), dict(x0=-2)):
+        kw = dict(x0=0, y0=0, pd=10, w=64, h=32)
+        kw.update(bad)
+        with pytest.raises(es.EsError):
+            es.frame_planes(f, kw["x0"], kw["y0"], kw["pd"], 0x4000, 0x8000, 0xC000, kw["w"], kw["h"])
+    f8 = es.frame_view("nv12", 0x1000, 0x2000, 0, 4096, 4096)
+    with pytest.raises(es.EsError, match="depth"):
+        es.frame_planes(es.frame_view("p010", 0x1000, 0x2000, 0, 4096, 4096), 0, 0, 8, 0x4000, 0x8000, 0xC000, 64, 32)
+    with pytest.raises(es.EsError, match="aligned"):
+        es.frame_planes(f8, 0, 0, 10, 0x4002, 0x8000, 0xC000, 64, 32)
+    with pytest.raises(es.EsError):
+        es.color_sums(0x1000, 0x2000, 10, 3, 0x3000)
