@@ -12,6 +12,7 @@
 //
 // Frames are read in place in their decoded formats (es_frame: P010, NV12 and planar 4:2:0 at 8 or 16 bits per
 // sample); with chroma planes each frame step also measures PSNR and the mean colours (src/main.cpp:543-556).
+#include "cache.hpp"
 #include "cambi_host.hpp"
 #include "energy.hpp"
 #include "frame.hpp"
@@ -87,7 +88,8 @@ public:
         up = 10 - c.depth;
     }
     virtual ~ScorerBase() = default;
-    virtual void submit(int n, const es_frame *src, const es_frame *encp, hipStream_t st) = 0;
+    // pts: the source frames' timestamps for the source-CAMBI caches, or null
+    virtual void submit(int n, const es_frame *src, const es_frame *encp, const int64_t *pts, hipStream_t st) = 0;
     virtual void sync() = 0;
     virtual void segment_hist(int e, int seg, Accum &a) { (void)e, (void)seg, (void)a; }
 
@@ -115,6 +117,24 @@ public:
     int window_base, up;
     long frames = 0;  // frame steps submitted
     int chroma = -1;  // whether the run's frames carry chroma planes (-1 before the first frame)
+    std::vector<std::unique_ptr<CambiCacheFile>> caches;  // per crop, when the run uses source-CAMBI caches
+    long cache_hits = 0;
+
+    void use_cache(const std::string &path, const std::string &dir)
+    {
+        caches.clear();
+        for (const Crop &cr : crops)
+            caches.push_back(std::make_unique<CambiCacheFile>(path, cr.cx, cr.cy, cr.w, cr.h, cfg.depth, dir));
+    }
+    // The cached CAMBI of crop c's source picture in the frame of timestamp pts.
+    bool cached_source(int c, int64_t pts, double *score) const
+    {
+        return !caches.empty() && caches[c]->find(pts, score);
+    }
+    void keep_source(int c, int64_t pts, double score)
+    {
+        if (!caches.empty()) caches[c]->add(pts, score);
+    }
 
     // The frames of a run as plain luma planes of the config's formats (es_scorer_submit).
     void luma_frames(int n, const void *const *src, const void *const *encp, std::vector<es_frame> &fs,
@@ -260,11 +280,11 @@ public:
             if (e) (void)hipEventDestroy(e);
     }
 
-    void submit(int n, const es_frame *src, const es_frame *encp, hipStream_t st) override
+    void submit(int n, const es_frame *src, const es_frame *encp, const int64_t *pts, hipStream_t st) override
     {
         check_frames(n, src, encp);
         for (int i = 0; i < n; i++) {
-            queue(frames, src[i], encp + (size_t)i * enc.size(), st);
+            queue(frames, src[i], encp + (size_t)i * enc.size(), pts ? pts[i] : INT64_MIN, st);
             if (frames > 0) collect(frames - 1);
             frames++;
         }
@@ -281,7 +301,10 @@ private:
     struct CropBufs {
         std::unique_ptr<CambiUnit> cambi[2];
         DevBuf ring;  // 13 mid planes by frame % 13
+        bool hit[2] = {};  // the source CAMBI of the frame in slot at came from the cache
+        double cached[2] = {};
     };
+    int64_t slot_pts[2] = {INT64_MIN, INT64_MIN};
     struct EncBufs {
         std::unique_ptr<CambiUnit> cambi[2];
         DevBuf mid;  // 7 mid planes by frame % 7
@@ -303,11 +326,13 @@ private:
     }
 
     // queue (src/main.cpp:516-574): the GPU work of frame f
-    void queue(long f, const es_frame &src, const es_frame *encp, hipStream_t st)
+    void queue(long f, const es_frame &src, const es_frame *encp, int64_t pts, hipStream_t st)
     {
         const int at = (int)(f % 2);
+        slot_pts[at] = pts;
         for (size_t c = 0; c < crops.size(); c++) {
-            if (!crop_active((int)c, f)) continue;
+            cb[c].hit[at] = cached_source((int)c, pts, &cb[c].cached[at]);
+            if (!crop_active((int)c, f) || cb[c].hit[at]) continue;
             const PlaneIn p = src_plane(src, (int)c);
             cb[c].cambi[at]->launch(p.ptr, p.pitch, p.bytes, p.shift, st, p.rshift);
         }
@@ -367,8 +392,16 @@ private:
         const int at = (int)(f % 2);
         hip_check(hipEventSynchronize(done[at]), "waiting for a frame");
         std::vector<double> src_score(crops.size(), 0.0);
-        for (size_t c = 0; c < crops.size(); c++)
-            if (crop_active((int)c, f)) src_score[c] = cb[c].cambi[at]->finish();
+        for (size_t c = 0; c < crops.size(); c++) {
+            if (!crop_active((int)c, f)) continue;
+            if (cb[c].hit[at]) {
+                src_score[c] = cb[c].cached[at];
+                cache_hits++;
+            } else {
+                src_score[c] = cb[c].cambi[at]->finish();
+                keep_source((int)c, slot_pts[at], src_score[c]);
+            }
+        }
         for (size_t e = 0; e < enc.size(); e++) {
             if (!active((int)e, f)) continue;
             EncodeState &s = enc[e];
@@ -589,14 +622,14 @@ public:
 
     // The rounds run on the scorer's own stream (a graph cannot be captured on the null stream), ordered after the
     // caller's earlier work on `st`, and the caller's later work on `st` waits for them.
-    void submit(int n, const es_frame *src, const es_frame *encp, hipStream_t st) override
+    void submit(int n, const es_frame *src, const es_frame *encp, const int64_t *pts, hipStream_t st) override
     {
         check_frames(n, src, encp);
         hip_check(hipEventRecord(joined, st), "event");
         hip_check(hipStreamWaitEvent(work, joined, 0), "event");
         for (int i = 0; i < n; i += B) {
             const int m = std::min(B, n - i);
-            round(m, src + i, encp + (size_t)i * enc.size(), work);
+            round(m, src + i, encp + (size_t)i * enc.size(), pts ? pts + i : nullptr, work);
         }
         hip_check(hipEventRecord(joined, work), "event");
         hip_check(hipStreamWaitEvent(st, joined, 0), "event");
@@ -633,6 +666,9 @@ private:
     struct Pending {
         long f0 = 0;
         int n = 0;
+        std::vector<int64_t> pts;      // per frame of the round
+        std::vector<char> hit;         // per (frame, crop): the source CAMBI came from the cache
+        std::vector<double> cached;
     };
 
     int B, R, P, roles, tile_h, cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
@@ -699,7 +735,7 @@ private:
                 CambiPic &pc = pics[pidx];
                 const int c = crop_of_role(r);
                 const CambiInfo &ci = crops[c].ci;
-                const bool on = i < n && (r < C ? crop_active(r, f) : active(r - C, f));
+                const bool on = i < n && (r < C ? crop_active(r, f) && !pend[p].hit[(size_t)i * C + r] : active(r - C, f));
                 for (int s = 0; s < kScales; s++) {
                     first[pidx * kScales + s] = t;
                     if (on) t += cvalue_tiles(ci.sw[s], ci.sh[s], tile_h);
@@ -883,10 +919,25 @@ private:
         hip_check(hipEventRecord(ev[p][ES_STAGES], st), "stage event");
     }
 
-    void round(int n, const es_frame *src, const es_frame *encp, hipStream_t st)
+    void round(int n, const es_frame *src, const es_frame *encp, const int64_t *pts, hipStream_t st)
     {
         const int p = (int)(rounds % 2);
         if (pend[p].n) fold(p);
+        const int C = (int)crops.size();
+        Pending &pd = pend[p];
+        pd.pts.assign(B, INT64_MIN);
+        pd.hit.assign((size_t)B * C, 0);
+        pd.cached.assign((size_t)B * C, 0.0);
+        for (int i = 0; i < n; i++) {
+            pd.pts[i] = pts ? pts[i] : INT64_MIN;
+            for (int c = 0; c < C; c++) {
+                double v;
+                if (crop_active(c, frames + i) && cached_source(c, pd.pts[i], &v)) {
+                    pd.hit[(size_t)i * C + c] = 1;
+                    pd.cached[(size_t)i * C + c] = v;
+                }
+            }
+        }
         fill(p, frames, n, src, encp);
         pipeline(p, st);
         hip_check(hipEventRecord(done[p], st), "event");
@@ -917,11 +968,18 @@ private:
         const int C = (int)crops.size(), E = (int)enc.size();
         for (int i = 0; i < pend[p].n; i++) {
             const long f = pend[p].f0 + i;
+            for (int c = 0; c < C; c++) {
+                if (!crop_active(c, f)) continue;
+                if (pend[p].hit[(size_t)i * C + c]) cache_hits++;
+                else keep_source(c, pend[p].pts[i], sc[i * roles + c]);
+            }
             for (int e = 0; e < E; e++) {
                 if (!active(e, f)) continue;
                 EncodeState &s = enc[e];
                 const size_t k = (size_t)i * E + e;
-                const double score = sc[i * roles + C + e], src = sc[i * roles + s.crop];
+                const size_t kc = (size_t)i * C + s.crop;
+                const double score = sc[i * roles + C + e];
+                const double src = pend[p].hit[kc] ? pend[p].cached[kc] : sc[i * roles + s.crop];
                 s.cambi.push_back(score);
                 s.cambi_src.push_back(src);
                 s.quantile.push_back(q[k]);
@@ -972,7 +1030,7 @@ int es_scorer_submit(es_scorer *s, int nframes, const void *const *src, const vo
         ensure_init();
         std::vector<es_frame> fs, fe;
         s->impl->luma_frames(nframes, src, enc, fs, fe);
-        s->impl->submit(nframes, fs.data(), fe.data(), hs(stream));
+        s->impl->submit(nframes, fs.data(), fe.data(), nullptr, hs(stream));
     });
 }
 
@@ -981,7 +1039,44 @@ int es_scorer_submit_frames(es_scorer *s, int nframes, const es_frame *src, cons
     return guard([&] {
         require(s && src && enc && nframes >= 0, "bad argument");
         ensure_init();
-        s->impl->submit(nframes, src, enc, hs(stream));
+        s->impl->submit(nframes, src, enc, nullptr, hs(stream));
+    });
+}
+
+int es_scorer_submit_pts(es_scorer *s, int nframes, const es_frame *src, const es_frame *enc, const int64_t *pts,
+                         es_stream stream)
+{
+    return guard([&] {
+        require(s && src && enc && pts && nframes >= 0, "bad argument");
+        ensure_init();
+        s->impl->submit(nframes, src, enc, pts, hs(stream));
+    });
+}
+
+int es_scorer_use_cache(es_scorer *s, const char *source_path, const char *dir)
+{
+    return guard([&] {
+        require(s && source_path, "null argument");
+        require(s->impl->frames == 0, "the cache must be chosen before the first frame");
+        s->impl->use_cache(source_path, dir ? dir : "");
+    });
+}
+
+int es_scorer_cache_hits(es_scorer *s, long *hits)
+{
+    return guard([&] {
+        require(s && hits, "null argument");
+        *hits = s->impl->cache_hits;
+    });
+}
+
+int es_scorer_cache_file(es_scorer *s, int crop, const char **file)
+{
+    return guard([&] {
+        require(s && file, "null argument");
+        ScorerBase &b = *s->impl;
+        require(crop >= 0 && crop < (int)b.crops.size(), "no such crop");
+        *file = b.caches.empty() ? "" : b.caches[crop]->file().c_str();
     });
 }
 
@@ -1183,11 +1278,11 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "creating a stream");
             std::vector<es_frame> fs, fe;
             sc.luma_frames(need, src, enc, fs, fe);
-            sc.submit(t.batch, fs.data(), fe.data(), st);
+            sc.submit(t.batch, fs.data(), fe.data(), nullptr, st);
             sc.sync();
             hip_check(hipStreamSynchronize(st), "tuning");
             const auto t0 = std::chrono::steady_clock::now();
-            sc.submit(2 * t.batch, fs.data() + t.batch, fe.data() + (size_t)t.batch * ne, st);
+            sc.submit(2 * t.batch, fs.data() + t.batch, fe.data() + (size_t)t.batch * ne, nullptr, st);
             sc.sync();
             hip_check(hipStreamSynchronize(st), "tuning");
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

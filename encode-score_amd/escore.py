@@ -107,6 +107,7 @@ def _sig(name, args, res=_i):
 
 
 _sig("es_last_error", [], C.c_char_p)
+_sig("es_cambi_cache_file", [_vp], C.c_char_p)
 for _n, _a in [
     ("es_init", [_i]), ("es_device_count", [C.POINTER(_i)]), ("es_device_synchronize", []),
     ("es_stream_create", [C.POINTER(_vp)]), ("es_stream_destroy", [_vp]), ("es_stream_synchronize", [_vp]),
@@ -147,6 +148,13 @@ for _n, _a in [
     ("es_scorer_set_vmaf", [_vp, _i, C.c_long, _vp]),
     ("es_scorer_write_json", [_vp, C.POINTER(RunInfo), C.c_char_p]),
     ("es_scorer_write_summary", [_vp, C.POINTER(RunInfo), C.c_double, C.c_char_p]),
+    ("es_cambi_cache_open", [C.c_char_p, _i, _i, _i, _i, _i, C.c_char_p, C.POINTER(_vp)]),
+    ("es_cambi_cache_find", [_vp, C.c_int64, _dp, C.POINTER(_i)]),
+    ("es_cambi_cache_add", [_vp, C.c_int64, C.c_double]), ("es_cambi_cache_close", [_vp]),
+    ("es_scorer_use_cache", [_vp, C.c_char_p, C.c_char_p]),
+    ("es_scorer_submit_pts", [_vp, _i, C.POINTER(Frame), C.POINTER(Frame), _vp, _vp]),
+    ("es_scorer_cache_hits", [_vp, C.POINTER(C.c_long)]),
+    ("es_scorer_cache_file", [_vp, _i, C.POINTER(C.c_char_p)]),
     ("es_synth_frame_host", [C.POINTER(SynthDesc), _i, _vp, _i, _i]),
 ]:
     _sig(_n, _a)
@@ -366,13 +374,32 @@ class Scorer:
         self.ptr = p.value
         self.n_encodes = len(encodes)
 
-    def submit_frames(self, src_frames, enc_frames, stream=None):
-        """src_frames: n Frame structures; enc_frames: n rows of n_encodes Frame structures."""
+    def submit_frames(self, src_frames, enc_frames, stream=None, pts=None):
+        """src_frames: n Frame structures; enc_frames: n rows of n_encodes Frame structures; pts: the source frames'
+        timestamps for the source-CAMBI cache (use_cache), or None."""
         n = len(src_frames)
         src = (Frame * n)(*src_frames)
         flat = [f for row in enc_frames for f in row]
         enc = (Frame * max(1, len(flat)))(*flat)
-        check(lib.es_scorer_submit_frames(self.ptr, n, src, enc, _ptr(stream)))
+        if pts is None:
+            check(lib.es_scorer_submit_frames(self.ptr, n, src, enc, _ptr(stream)))
+        else:
+            p = np.ascontiguousarray(pts, np.int64)
+            check(lib.es_scorer_submit_pts(self.ptr, n, src, enc, p.ctypes.data, _ptr(stream)))
+
+    def use_cache(self, source_path, directory=None):
+        check(lib.es_scorer_use_cache(self.ptr, str(source_path).encode(),
+                                      None if directory is None else str(directory).encode()))
+
+    def cache_hits(self):
+        n = C.c_long()
+        check(lib.es_scorer_cache_hits(self.ptr, C.byref(n)))
+        return n.value
+
+    def cache_file(self, crop=0):
+        f = C.c_char_p()
+        check(lib.es_scorer_cache_file(self.ptr, crop, C.byref(f)))
+        return f.value.decode()
 
     def psnr(self, e, n):
         out = np.empty((n, 3))
@@ -487,6 +514,41 @@ def psnr(sse, n, depth):
     out = C.c_double()
     check(lib.es_psnr(int(sse), int(n), depth, C.byref(out)))
     return out.value
+
+
+NOPTS = -(1 << 63)
+
+
+class CambiCache:
+    """The source-CAMBI cache (CambiCache, src/cache.hpp) over es_cambi_cache_*."""
+
+    def __init__(self, path, x0, y0, w, h, depth, directory=None):
+        p = _vp()
+        check(lib.es_cambi_cache_open(str(path).encode(), x0, y0, w, h, depth,
+                                      None if directory is None else str(directory).encode(), C.byref(p)))
+        self.ptr = p.value
+
+    def find(self, pts):
+        v, found = C.c_double(), _i()
+        check(lib.es_cambi_cache_find(self.ptr, pts, C.byref(v), C.byref(found)))
+        return v.value if found.value else None
+
+    def add(self, pts, score):
+        check(lib.es_cambi_cache_add(self.ptr, pts, score))
+
+    def file(self):
+        return lib.es_cambi_cache_file(self.ptr).decode()
+
+    def close(self):
+        if self.ptr:
+            check(lib.es_cambi_cache_close(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def copy_kernel(dst, src, nbytes, stream=None):

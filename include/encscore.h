@@ -169,6 +169,26 @@ int es_color_values(const uint64_t *sums, size_t n, double *out);
  * 2^depth - 1, capped at 6 depth + 12 dB. */
 int es_psnr(uint64_t sse, size_t n, int depth, double *out);
 
+/* ---- the source-CAMBI cache (replaces class CambiCache, src/cache.hpp:9-29, src/cache.cpp:14-88) ---- */
+
+#define ES_NOPTS INT64_MIN /* a frame without a timestamp (AV_NOPTS_VALUE): never found, never kept */
+typedef struct es_cambi_cache_s es_cambi_cache;
+/* CambiCache(path, x0, y0, w, h, depth): the per-frame CAMBI of the source file `path` cropped to w x h at (x0, y0)
+ * and scored at `depth` bits, kept in `dir` (NULL: $XDG_CACHE_HOME/encode-score, else $HOME/.cache/encode-score).
+ * The file format and naming are the reference's (16-byte {int64 pts, double score} records; FNV-1a of the key and
+ * the source's first and last MiB) with this library's own version in the key, so the scores of a different CAMBI
+ * implementation are never read as this one's.  When the source or the directory cannot be used the cache is inert:
+ * it finds nothing and keeps nothing. */
+int es_cambi_cache_open(const char *path, int x0, int y0, int w, int h, int depth, const char *dir,
+                        es_cambi_cache **out);
+/* CambiCache::find: *found = 1 and *score when the frame of timestamp pts has a cached score. */
+int es_cambi_cache_find(const es_cambi_cache *c, int64_t pts, double *score, int *found);
+/* CambiCache::add: keeps the score (written to the file every 240 records and on close). */
+int es_cambi_cache_add(es_cambi_cache *c, int64_t pts, double score);
+/* The cache file's path ("" when inert), valid until es_cambi_cache_close. */
+const char *es_cambi_cache_file(const es_cambi_cache *c);
+int es_cambi_cache_close(es_cambi_cache *c);
+
 /* ---- the whole measure path of one source against its encodes (src/main.cpp:516-643 queue/collect,
  *      src/stats.hpp Accum) ---- */
 
@@ -245,6 +265,17 @@ int es_scorer_submit(es_scorer *s, int nframes, const void *const *src, const vo
  * the PSNR of luma and both chroma planes and the mean colour of both pictures (src/main.cpp:525, :543-556,
  * :594-605); every frame of a run must carry chroma, or none.  Crops and picture sizes must be even. */
 int es_scorer_submit_frames(es_scorer *s, int nframes, const es_frame *src, const es_frame *enc, es_stream stream);
+/* Uses source-CAMBI caches of the source file `source_path` (one per crop, keyed as CambiCache keys them,
+ * src/main.cpp:454-455; dir as for es_cambi_cache_open): frames submitted with timestamps whose source CAMBI is
+ * cached skip the source's CAMBI, and the scores computed for the others are added (src/main.cpp:520-521, :585-587). */
+int es_scorer_use_cache(es_scorer *s, const char *source_path, const char *dir);
+/* As es_scorer_submit_frames, with the source frames' timestamps pts[i] (ES_NOPTS for none). */
+int es_scorer_submit_pts(es_scorer *s, int nframes, const es_frame *src, const es_frame *enc, const int64_t *pts,
+                         es_stream stream);
+/* Source pictures whose CAMBI came from the cache so far, and the cache file of each crop (as for es_cambi_cache_file;
+ * crop k is the k-th distinct encode size). */
+int es_scorer_cache_hits(es_scorer *s, long *hits);
+int es_scorer_cache_file(es_scorer *s, int crop, const char **file);
 /* Waits for the queued work and folds its figures into the segments. */
 int es_scorer_sync(es_scorer *s);
 int es_scorer_frames(es_scorer *s, long *frames);

@@ -361,6 +361,37 @@ def accum_summary(a, vmaf_total=float("nan")):
     return out, hists
 
 
+# ---- the source-CAMBI cache's file naming and records (src/cache.cpp:14-59) ----
+
+FNV_OFFSET, FNV_PRIME = 0xcbf29ce484222325, 0x100000001b3
+
+
+def fnv1a64(data, h=FNV_OFFSET):
+    """64-bit FNV-1a (Fowler, Noll, Vo): for each byte, xor then multiply by the FNV prime, modulo 2^64."""
+    for b in bytes(data):
+        h = ((h ^ b) * FNV_PRIME) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def cambi_cache_name(path, version, x0, y0, w, h, depth, sample=1 << 20):
+    """The cache file's name: FNV-1a of "cambi <version> <size> <x0> <y0> <w> <h> <depth>", then of the source's first
+    and last `sample` bytes (the whole file twice when it is shorter)."""
+    data = open(path, "rb").read()
+    size = len(data)
+    hsh = fnv1a64(f"cambi {version} {size} {x0} {y0} {w} {h} {depth}".encode())
+    hsh = fnv1a64(data[:sample], hsh)
+    hsh = fnv1a64(data[max(size - sample, 0):][:sample], hsh)
+    return f"{hsh:016x}.cambi"
+
+
+def cambi_cache_records(file):
+    """The whole 16-byte {int64 pts, double score} records of a cache file, as (pts, score) pairs."""
+    raw = open(file, "rb").read()
+    n = len(raw) // 16
+    rec = np.frombuffer(raw[:n * 16], dtype=[("pts", "<i8"), ("score", "<f8")])
+    return [(int(a), float(b)) for a, b in rec]
+
+
 def ref_stats():
     """The reference's own stats.hpp behind oracle/_ref's shim, or None when it was not built."""
     if not os.path.exists(REF_STATS):

@@ -151,7 +151,7 @@ def compare(got, ref_summary):
             assert (isinstance(v, float) and np.isnan(v) and np.isnan(g)) or g == v, k
 
 
-def submit_formats(gpu, sc, src, enc, src_fmt, enc_fmt, junk=None, chunks=(9,)):
+def submit_formats(gpu, sc, src, enc, src_fmt, enc_fmt, junk=None, chunks=(9,), pts=None):
     keep, fs, fe = [], [], []
     for s, e in zip(src, enc):
         f1, d1 = upload(gpu, src_fmt, pack(src_fmt, *s, pad=3, junk=junk))
@@ -161,7 +161,7 @@ def submit_formats(gpu, sc, src, enc, src_fmt, enc_fmt, junk=None, chunks=(9,)):
         fe.append([f2])
     i = 0
     for c in chunks + (len(fs),):
-        sc.submit_frames(fs[i:i + c], fe[i:i + c])
+        sc.submit_frames(fs[i:i + c], fe[i:i + c], pts=None if pts is None else pts[i:i + c])
         i += c
         if i >= len(fs):
             break
@@ -309,3 +309,32 @@ def test_json_report_in_the_reference_format(gpu, tmp_path):
     assert sum(1 for x in lines if x == "  worst segments:") == 2
     assert any(re.match(r"  00:00:1\d       score ", x) for x in lines)
     del keep
+
+
+@pytest.mark.parametrize("mode", ["plain", "batched"])
+def test_source_cambi_cache_skips_cached_frames(gpu, orc, tmp_path, mode):
+    """A second run against the same source reads every source CAMBI from the cache (src/main.cpp:520-521, :585-587)
+    and skips computing it, with the same figures; the first run's file holds its scores as 16-byte records."""
+    n, w, h = 12, 224, 128
+    print(f"case cache {mode}", flush=True)
+    src, enc, crop = yuv_run(gpu, n, w, h, seed=46)
+    source = tmp_path / "source.mkv"
+    source.write_bytes(b"a source file" * 1000)
+    pts = np.arange(n, dtype=np.int64) * 1001 + 7
+    m = gpu.ES_PLAIN if mode == "plain" else gpu.ES_BATCHED
+    results = []
+    for k in range(3):
+        sc = gpu.Scorer(m, W, H, [gpu.EncodeDesc(w, h, n, w, 2, 0)], batch=5)
+        if k:
+            sc.use_cache(source, tmp_path / "cache")
+        keep = submit_formats(gpu, sc, src, enc, "yuv420p10", "yuv420p10", chunks=(7,), pts=pts if k else None)
+        results.append((sc.cambi(0, n), sc.segment(0, -1), sc.psnr(0, n), sc.cache_hits(), sc.cache_file(0)))
+        del sc, keep
+    (a0, b0), seg0, p0, _, _ = results[0]
+    for k in (1, 2):
+        (a, b), seg, p, hits, file = results[k]
+        assert hits == (0 if k == 1 else n), k
+        assert np.array_equal(a, a0) and np.array_equal(b, b0) and np.array_equal(p, p0)
+        compare(seg, ({key: seg0[key] for key in seg0}, None))
+    assert results[1][4] == results[2][4] != ""
+    assert orc.cambi_cache_records(results[1][4]) == [(int(t), float(v)) for t, v in zip(pts, b0)]
