@@ -126,6 +126,7 @@ def main():
     ap.add_argument("--tile-h", type=int, default=0)
     ap.add_argument("--cv-bmax", type=int, default=0, help="c-value LDS layout, 32 or 64 (with --tune 0)")
     ap.add_argument("--cv-waves", type=int, default=0, help="persistent c-value waves per CU (with --tune 0)")
+    ap.add_argument("--cv-tile-blocks", type=int, default=0, help="c-value kernel as one block per tile (with --tune 0)")
     ap.add_argument("--plain-steps", type=int, default=48)
     ap.add_argument("--cpu-frames", type=int, default=-1, help="frames of the CPU baseline sample (-1: per config)")
     ap.add_argument("--pool", type=int, default=512, help="distinct frames generated per file at most")
@@ -135,6 +136,8 @@ def main():
     ap.add_argument("--tune-cache", default=None, help="JSON file to keep autotuner results in")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run the calibration copy kernel over this many MiB (for HBM byte counters)")
+    ap.add_argument("--serial", type=int, default=0,
+                    help="run each round's CAMBI and energy chains one after the other (1) instead of overlapped (0)")
     ap.add_argument("--chroma", type=int, default=1,
                     help="frames carry chroma planes: PSNR and mean colours in every step (1) or luma only (0)")
     ap.add_argument("--ingest", type=int, default=50,
@@ -222,20 +225,23 @@ def main():
 
     # the autotuner, on the pool's frames, outside the timed region
     tuned = None
-    batch, tile_h, cv_bmax, cv_waves = args.batch, args.tile_h, args.cv_bmax, args.cv_waves
+    batch, tile_h, cv_bmax, cv_waves, cv_tb = args.batch, args.tile_h, args.cv_bmax, args.cv_waves, bool(args.cv_tile_blocks)
     if args.tune:
         n_tune = min(pool, 96)
         s_t = [src_frames[f].ptr for f in range(n_tune)]
         e_t = [[enc_frames[i][f].ptr for i in range(len(encs))] for f in range(n_tune)]
         t0 = time.perf_counter()
-        tuned = es.autotune(W, H, descs, s_t, e_t, src_bytes=sbytes, src_shift=sshift, cache=args.tune_cache)
+        tuned = es.autotune(W, H, descs, s_t, e_t, src_bytes=sbytes, src_shift=sshift, cache=args.tune_cache,
+                            serial_stages=bool(args.serial))
         tuned["seconds"] = round(time.perf_counter() - t0, 2)
         batch, tile_h, cv_bmax, cv_waves = tuned["batch"], tuned["tile_h"], tuned["cv_bmax"], tuned["cv_waves"]
+        cv_tb = tuned["cv_tile_blocks"]
         print(f"[bench] autotuner: {tuned}", file=sys.stderr, flush=True)
 
     # batched form (graph replay), with live per-stage events
     bat = es.Scorer(es.ES_BATCHED, W, H, descs, batch=batch, tile_h=tile_h, use_graph=True, profile=True,
-                    src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves)
+                    src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves, serial_stages=bool(args.serial),
+                    cv_tile_blocks=cv_tb)
     submit(bat, 0, args.warmup)
     bat.sync()
     warm_stage, warm_rounds = bat.stage_ms()
@@ -290,8 +296,10 @@ def main():
         "dtype": "u16",
         "data": "synthetic",
         "config": {"workload": workload, "config": args.config, "width": W, "height": H, "encodes": len(encs),
-                   "batch": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32, "cv_waves": cv_waves or 16,
-                   "chroma": bool(args.chroma), "form": "batched, integer energies, graph replay"},
+                   "batch": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32,
+                   "cv_waves": "one block per tile" if cv_tb else (cv_waves or 16),
+                   "chroma": bool(args.chroma), "stages": "serial" if args.serial else "CAMBI and energy chains overlapped",
+                   "form": "batched, integer energies, graph replay"},
         "autotuner": tuned,
         "encode_frames_per_s": round(encode_frames / wall, 3),
         "device_ms_per_step": round(dev * 1e3 / args.steps, 4),

@@ -449,6 +449,7 @@ public:
         cv_blocks = std::max(1, cus) * (c.cv_waves > 0 ? c.cv_waves : 16);
         require(c.cv_bmax == 0 || c.cv_bmax == 32 || c.cv_bmax == 64, "cv_bmax must be 32 or 64");
         cv_bmax = c.cv_bmax ? c.cv_bmax : 32;
+        cv_tile_blocks = c.cv_tile_blocks != 0;
         const int C = (int)crops.size(), E = (int)enc.size();
         roles = C + E;
         P = B * roles;
@@ -585,12 +586,17 @@ public:
             hip_check(hipEventCreateWithFlags(&done[k], hipEventDisableTiming), "creating an event");
         }
         hip_check(hipStreamCreateWithFlags(&work, hipStreamNonBlocking), "creating a stream");
+        hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "creating a stream");
         hip_check(hipEventCreateWithFlags(&joined, hipEventDisableTiming), "creating an event");
+        hip_check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "creating an event");
+        hip_check(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), "creating an event");
         use_graph = c.use_graph != 0;
         profile = c.profile != 0;
+        serial = c.serial_stages != 0;
         if (profile)
             for (auto &row : ev)
-                for (auto &e : row) hip_check(hipEventCreate(&e), "creating an event");
+                for (auto &pair : row)
+                    for (auto &e : pair) hip_check(hipEventCreate(&e), "creating an event");
     }
 
     ~BatchedScorer() override
@@ -607,14 +613,17 @@ public:
                 if (sgraph[k][j]) (void)hipGraphDestroy(sgraph[k][j]);
             }
         }
-        if (work) {
-            (void)hipStreamSynchronize(work);
-            (void)hipStreamDestroy(work);
-        }
-        if (joined) (void)hipEventDestroy(joined);
+        for (hipStream_t *q : {&work, &side})
+            if (*q) {
+                (void)hipStreamSynchronize(*q);
+                (void)hipStreamDestroy(*q);
+            }
+        for (hipEvent_t e : {joined, fork_ev, join_ev})
+            if (e) (void)hipEventDestroy(e);
         for (auto &row : ev)
-            for (auto &e : row)
-                if (e) (void)hipEventDestroy(e);
+            for (auto &pair : row)
+                for (auto &e : pair)
+                    if (e) (void)hipEventDestroy(e);
     }
 
     double stage_ms[ES_STAGES] = {};
@@ -672,6 +681,7 @@ private:
     };
 
     int B, R, P, roles, tile_h, cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
+    bool cv_tile_blocks = false;
     std::vector<long long> max_px;
     std::vector<int> max_strips;
     struct Layout {
@@ -689,10 +699,11 @@ private:
     size_t lout_words = 0;
     size_t desc_bytes, res_bytes;
     HostBuf stage_buf[2], res[2];
-    hipEvent_t done[2] = {}, joined = nullptr;
-    hipEvent_t ev[2][ES_STAGES + 1] = {};
+    hipEvent_t done[2] = {}, joined = nullptr, fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t ev[2][ES_STAGES][2] = {};  // profiled: each stage's start and end on the stream it runs on
     bool profile = false;
-    hipStream_t work = nullptr;
+    hipStream_t work = nullptr, side = nullptr;  // the round's main stream and the energy chain's
+    bool serial = false;
     hipGraph_t graph[2] = {}, sgraph[2][ES_STAGES] = {};
     hipGraphExec_t exec[2] = {}, sexec[2][ES_STAGES] = {};
     Pending pend[2];
@@ -856,8 +867,10 @@ private:
             for (int s = 1; s < kScales; s++) launch_cambi_scale(pics, P, max_strips[s], s, st);
             break;
         case 3:
-            launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, 0, tile_h, 1, counts.as<unsigned>() + (size_t)P * kScales,
-                                 cv_blocks, cv_bmax, st);
+            // persistent waves taking tiles from a counter, or one block per tile of the round's largest layout (the
+            // blocks past the round's tiles exit at once)
+            launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, (int)seg_off[P], tile_h, cv_tile_blocks ? 0 : 1,
+                                 counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st);
             break;
         case 4:
             for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, kTopkBlocks, st);
@@ -894,29 +907,57 @@ private:
         return e;
     }
 
-    // The round on `st`: eager launches, one graph, or (profiled) one graph per stage with events between them.
+    // The order of the round's stages: the descriptor upload first; then the CAMBI chain (stages 1-5) on `st` and the
+    // energy chain (6-9: window stats, frame sums, lasting texture, collect) on the side stream, which need nothing
+    // from each other and overlap (the c-value kernel is VALU-bound, the energy kernels HBM-bound); the results copy
+    // after both.  With serial set, all on `st` in order.
+    template <class F>
+    void schedule(hipStream_t st, F &&run)
+    {
+        run(0, st);
+        if (serial) {
+            for (int k = 1; k < ES_STAGES; k++) run(k, st);
+            return;
+        }
+        hip_check(hipEventRecord(fork_ev, st), "event");
+        hip_check(hipStreamWaitEvent(side, fork_ev, 0), "event");
+        for (int k = 1; k <= 5; k++) run(k, st);
+        for (int k = 6; k <= 9; k++) run(k, side);
+        hip_check(hipEventRecord(join_ev, side), "event");
+        hip_check(hipStreamWaitEvent(st, join_ev, 0), "event");
+        run(10, st);
+    }
+
+    // The round on `st`: eager launches, one graph (both chains as branches), or (profiled) one graph per stage with
+    // events around each on its stream.
     void pipeline(int p, hipStream_t st)
     {
+        auto timed = [&](int k, hipStream_t s, auto &&launch) {
+            if (profile) hip_check(hipEventRecord(ev[p][k][0], s), "stage event");
+            launch();
+            if (profile) hip_check(hipEventRecord(ev[p][k][1], s), "stage event");
+        };
         if (!use_graph) {
-            for (int k = 0; k < ES_STAGES; k++) {
-                if (profile) hip_check(hipEventRecord(ev[p][k], st), "stage event");
-                stage(k, p, st);
-            }
-            if (profile) hip_check(hipEventRecord(ev[p][ES_STAGES], st), "stage event");
+            schedule(st, [&](int k, hipStream_t s) { timed(k, s, [&] { stage(k, p, s); }); });
             return;
         }
         if (!profile) {
-            if (!exec[p]) exec[p] = capture(0, ES_STAGES, p, st, &graph[p]);
+            if (!exec[p]) {
+                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
+                schedule(st, [&](int k, hipStream_t s) { stage(k, p, s); });
+                hip_check(hipStreamEndCapture(st, &graph[p]), "graph capture");
+                hip_check(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "graph instantiate");
+            }
             hip_check(hipGraphLaunch(exec[p], st), "graph launch");
             return;
         }
-        for (int k = 0; k < ES_STAGES; k++) {
-            hip_check(hipEventRecord(ev[p][k], st), "stage event");
-            if (k == 7 && chroma != 1) continue;  // no work in the stage: nothing to capture
-            if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, st, &sgraph[p][k]);
-            hip_check(hipGraphLaunch(sexec[p][k], st), "graph launch");
-        }
-        hip_check(hipEventRecord(ev[p][ES_STAGES], st), "stage event");
+        schedule(st, [&](int k, hipStream_t s) {
+            timed(k, s, [&] {
+                if (k == 7 && chroma != 1) return;  // no work in the stage: nothing to capture
+                if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, s, &sgraph[p][k]);
+                hip_check(hipGraphLaunch(sexec[p][k], s), "graph launch");
+            });
+        });
     }
 
     void round(int n, const es_frame *src, const es_frame *encp, const int64_t *pts, hipStream_t st)
@@ -953,7 +994,7 @@ private:
         if (profile) {
             for (int k = 0; k < ES_STAGES; k++) {
                 float ms = 0;
-                hip_check(hipEventElapsedTime(&ms, ev[p][k], ev[p][k + 1]), "stage time");
+                hip_check(hipEventElapsedTime(&ms, ev[p][k][0], ev[p][k][1]), "stage time");
                 stage_ms[k] += ms;
             }
             profiled_rounds++;
@@ -1263,7 +1304,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         base.use_graph = 1;
         base.profile = 0;
         es_tune_result best{base.batch ? base.batch : 16, base.tile_h ? base.tile_h : 32, base.cv_bmax ? base.cv_bmax : 32,
-                            base.cv_waves ? base.cv_waves : 16, 0.0, 0};
+                            base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks};
         const int ne = base.n_encodes;
         auto trial = [&](const es_tune_result &t) -> double {
             es_scorer_config c = base;
@@ -1271,6 +1312,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             c.tile_h = t.tile_h;
             c.cv_bmax = t.cv_bmax;
             c.cv_waves = t.cv_waves;
+            c.cv_tile_blocks = t.cv_tile_blocks;
             const int need = 3 * t.batch;
             if (nframes < need) return -1.0;  // too few frames for this batch
             BatchedScorer sc(c);
@@ -1298,7 +1340,8 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             for (int v : values) {
                 es_tune_result t = keep;
                 set(t, v);
-                if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves)
+                if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves &&
+                    t.cv_tile_blocks == keep.cv_tile_blocks)
                     continue;
                 const double ms = trial(t);
                 if (ms > 0 && ms < best.ms_per_step) {
@@ -1310,7 +1353,9 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         // tile height and LDS layout together: taller tiles see more values, which favours the wider passes
         try_set([](es_tune_result &t, int v) { t.tile_h = v % 100, t.cv_bmax = v / 100; },
                 {3216, 3224, 3232, 3248, 3264, 6416, 6424, 6432, 6448, 6464});
-        try_set([](es_tune_result &t, int v) { t.cv_waves = v; }, {8, 12, 16, 24});
+        // with the energy chain overlapping, fewer resident c-value waves can leave it room
+        // (0: one block per tile, which the hardware interleaves with the overlapping energy chain)
+        try_set([](es_tune_result &t, int v) { t.cv_waves = v, t.cv_tile_blocks = v == 0; }, {0, 6, 8, 12, 16, 24});
         try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32});
         const int trials = out->trials;
         *out = best;

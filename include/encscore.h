@@ -218,6 +218,10 @@ typedef struct {
     int profile;      /* batched form: time each stage of every round with events (es_scorer_stage_ms) */
     int cv_bmax;      /* c-value LDS layout: numbers per pass, 32 or 64 (0 = 32) */
     int cv_waves;     /* batched form: persistent c-value waves per CU (0 = 16) */
+    int serial_stages; /* batched form: run the CAMBI and energy chains of a round one after the other (0: overlap them
+                        * on two streams) */
+    int cv_tile_blocks; /* batched form: launch the c-value kernel as one block per tile instead of persistent waves,
+                         * which lets the hardware interleave it with the overlapping energy chain */
 } es_scorer_config;
 
 /* One segment of one encode (Accum, src/stats.hpp:36-82), with its histograms' quantiles and the score parts.
@@ -248,11 +252,12 @@ typedef struct {
     int batch, tile_h, cv_bmax, cv_waves;
     double ms_per_step;
     int trials;
+    int cv_tile_blocks;
 } es_tune_result;
 /* Autotuner (no counterpart in the reference): times the batched form of cfg's workload on the caller's frames
  * (nframes steps as for es_scorer_submit, at least three batches of the largest batch tried), trying the c-value
- * tile height with the c-value LDS layout (together), then the c-value waves per CU, then the batch, and returns the
- * fastest.  Each trial allocates a scorer of its own. */
+ * tile height with the c-value LDS layout (together), then the c-value waves per CU or one block per tile, then the
+ * batch, and returns the fastest.  Each trial allocates a scorer of its own. */
 int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *const *src, const void *const *enc,
                        es_tune_result *out);
 int es_scorer_destroy(es_scorer *s);
@@ -307,7 +312,8 @@ int es_scorer_write_json(es_scorer *s, const es_run_info *info, const char *path
  * each encode, after a header with the wall time `seconds`; to `path`, or to standard output when path is NULL. */
 int es_scorer_write_summary(es_scorer *s, const es_run_info *info, double seconds, const char *path);
 /* Batched form with profile set: the device milliseconds of each stage summed over the synced rounds, for
- * ES_STAGES stages in the order ES_STAGE_NAMES lists, and the number of rounds. */
+ * ES_STAGES stages in the order ES_STAGE_NAMES lists, and the number of rounds.  Each stage is timed on its own
+ * stream, so with overlapping chains the stages' times add up to more than the round's. */
 #define ES_STAGES 11
 #define ES_STAGE_NAMES "upload,cambi_base,cambi_scale,cambi_cvalues,cambi_topk,cambi_finish,window_stats,frame_sums,persist,collect,results"
 int es_scorer_stage_ms(es_scorer *s, double *ms, long *rounds);

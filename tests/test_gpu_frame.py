@@ -125,8 +125,8 @@ def yuv_run(gpu, n, w, h, seed, src_fmt="yuv420p10", enc_fmt="yuv420p10"):
     return src, enc, (cx, cy)
 
 
-def oracle_yuv(orc, src, enc, crop, w, h, per_segment, src_up=0):
-    run = orc.Run(w, h, per_segment=per_segment, threads=8)
+def oracle_yuv(orc, src, enc, crop, w, h, per_segment, src_up=0, depth=10):
+    run = orc.Run(w, h, depth=depth, up=10 - depth, per_segment=per_segment, threads=8)
     cx, cy = crop
     cam, cam_src, ps = [], [], []
     for s, e in zip(src, enc):
@@ -207,6 +207,25 @@ def test_scorer_reads_p010_and_nv12_in_place(gpu, orc):
             assert np.array_equal(sc.psnr(0, n), ps)
             compare(sc.segment(0, 0), orc.accum_summary(run.segment(0)))
             del keep
+
+
+def test_scorer_8bit_pipeline_matches_oracle(gpu, orc):
+    """Every file 8-bit: pipeline depth 8 (src/main.cpp:382), CAMBI's 8-bit promotion, the energies and colour at the
+    10-bit scale (up = 2), PSNR with peak 255 and cap 60 dB."""
+    n, w, h = 14, 224, 128
+    src, enc, crop = yuv_run(gpu, n, w, h, seed=47, src_fmt="yuv420p", enc_fmt="nv12")
+    run, cam, cam_src, ps = oracle_yuv(orc, src, enc, crop, w, h, 6, depth=8)
+    for mode in (gpu.ES_PLAIN, gpu.ES_BATCHED):
+        print(f"case scorer 8-bit pipeline {mode}", flush=True)
+        sc = gpu.Scorer(mode, W, H, [gpu.EncodeDesc(w, h, n, w, 1, 0)], depth=8, per_segment=6, batch=4,
+                        use_graph=mode == gpu.ES_BATCHED)
+        keep = submit_formats(gpu, sc, src, enc, "yuv420p", "nv12", chunks=(6,))
+        a, b = sc.cambi(0, n)
+        assert np.array_equal(a, cam) and np.array_equal(b, cam_src), mode
+        assert np.array_equal(sc.psnr(0, n), ps)
+        for s in range(run.segments()):
+            compare(sc.segment(0, s), orc.accum_summary(run.segment(s)))
+        del keep
 
 
 def test_frames_without_chroma_and_mixing_is_an_error(gpu):

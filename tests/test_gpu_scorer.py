@@ -179,10 +179,12 @@ def test_batched_larger_windows_match_oracle(gpu, orc, win):
     assert np.array_equal(hist, ref[1])
 
 
-@pytest.mark.parametrize("bmax,tile_h,waves", [(64, 32, 16), (64, 16, 8), (32, 48, 24), (32, 64, 12)])
+@pytest.mark.parametrize("bmax,tile_h,waves", [(64, 32, 16), (64, 16, 8), (32, 48, 24), (32, 64, 12), (32, 32, 0),
+                                               (64, 24, 0)])
 def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves):
-    """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, waves per CU)
-    gives the oracle's results; textured 640x360 frames have tiles with more than 32 relevant values."""
+    """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, waves per CU or
+    one block per tile (waves 0)) gives the oracle's results; textured 640x360 frames have tiles with more than 32
+    relevant values."""
     n, w, h = 14, 640, 360
     print(f"case layout bmax {bmax} tile_h {tile_h} waves {waves}", flush=True)
     src, enc = frames_of(gpu, n, w=w, h=h, seed=41, full=(w, h))
@@ -190,7 +192,7 @@ def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves):
     ds = [gpu.DeviceArray.from_host(f) for f in src]
     de = [gpu.DeviceArray.from_host(f) for f in enc]
     sc = gpu.Scorer(gpu.ES_BATCHED, w, h, [gpu.EncodeDesc(w, h, n, w, 2, 0)], batch=8, use_graph=True, tile_h=tile_h,
-                    cv_bmax=bmax, cv_waves=waves)
+                    cv_bmax=bmax, cv_waves=waves, cv_tile_blocks=waves == 0)
     sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
     sc.sync()
     a, b = sc.cambi(0, n)
@@ -210,10 +212,34 @@ def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
     cache = str(tmp_path / "tune.json")
     r = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
     assert r["batch"] in (8, 16, 32) and r["tile_h"] in (16, 24, 32, 48, 64)
-    assert r["cv_bmax"] in (32, 64) and r["cv_waves"] in (8, 12, 16, 24)
+    assert r["cv_bmax"] in (32, 64) and r["cv_waves"] in (0, 6, 8, 12, 16, 24)
+    assert r["cv_tile_blocks"] == (r["cv_waves"] == 0)
     assert r["ms_per_step"] > 0 and r["trials"] >= 14 and not r["cached"]
     again = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
     assert again["cached"] and {k: again[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")} == \
         {k: r[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")}
     with pytest.raises(gpu.EsError):  # too few frames for three batches
         gpu.autotune(W, H, descs, [d.ptr for d in ds[:10]], [[d.ptr] for d in de[:10]])
+
+
+def test_overlapped_and_serial_stages_agree(gpu):
+    """The CAMBI and energy chains overlapped on two streams (the default) give the figures of the serial order, in
+    eager, graph and profiled-graph forms."""
+    n = 24
+    src, enc = frames_of(gpu, n, seed=8)
+    ds = [gpu.DeviceArray.from_host(f) for f in src]
+    de = [gpu.DeviceArray.from_host(f) for f in enc]
+    out = []
+    for serial, graph, profile in ((True, True, False), (False, False, False), (False, True, False), (False, True, True)):
+        sc = gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], batch=5, use_graph=graph,
+                        profile=profile, serial_stages=serial, per_segment=10)
+        sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
+        sc.sync()
+        out.append((sc.cambi(0, n), [sc.segment(0, s, with_hist=True) for s in range(sc.segments(0))]))
+    (a0, b0), segs0 = out[0]
+    for (a, b), segs in out[1:]:
+        assert np.array_equal(a, a0) and np.array_equal(b, b0)
+        for (s0, h0), (s1, h1) in zip(segs0, segs):
+            assert np.array_equal(h0, h1)
+            for k, v in s0.items():
+                assert (v == s1[k]) or (isinstance(v, float) and np.isnan(v) and np.isnan(s1[k])), k
