@@ -593,6 +593,7 @@ public:
         use_graph = c.use_graph != 0;
         profile = c.profile != 0;
         serial = c.serial_stages != 0;
+        early = c.early_energies != 0;
         if (profile)
             for (auto &row : ev)
                 for (auto &pair : row)
@@ -703,7 +704,7 @@ private:
     hipEvent_t ev[2][ES_STAGES][2] = {};  // profiled: each stage's start and end on the stream it runs on
     bool profile = false;
     hipStream_t work = nullptr, side = nullptr;  // the round's main stream and the energy chain's
-    bool serial = false;
+    bool serial = false, early = false;
     hipGraph_t graph[2] = {}, sgraph[2][ES_STAGES] = {};
     hipGraphExec_t exec[2] = {}, sexec[2][ES_STAGES] = {};
     Pending pend[2];
@@ -909,8 +910,8 @@ private:
 
     // The order of the round's stages: the descriptor upload first; then the CAMBI chain (stages 1-5) on `st` and the
     // energy chain (6-9: window stats, frame sums, lasting texture, collect) on the side stream, which need nothing
-    // from each other and overlap (the c-value kernel is VALU-bound, the energy kernels HBM-bound); the results copy
-    // after both.  With serial set, all on `st` in order.
+    // from each other: the energies overlap the c-value kernel (VALU-bound while they are HBM-bound); the results
+    // copy after both.  With serial set, all on `st` in order.
     template <class F>
     void schedule(hipStream_t st, F &&run)
     {
@@ -919,9 +920,13 @@ private:
             for (int k = 1; k < ES_STAGES; k++) run(k, st);
             return;
         }
+        // the energy chain starts once the CAMBI pyramid is built (base and scale are HBM-bound like it, the c-value
+        // kernel is not), or with it when the energies are long next to the c-values (early)
+        const int fork_after = early ? 0 : 2;
+        for (int k = 1; k <= fork_after; k++) run(k, st);
         hip_check(hipEventRecord(fork_ev, st), "event");
         hip_check(hipStreamWaitEvent(side, fork_ev, 0), "event");
-        for (int k = 1; k <= 5; k++) run(k, st);
+        for (int k = fork_after + 1; k <= 5; k++) run(k, st);
         for (int k = 6; k <= 9; k++) run(k, side);
         hip_check(hipEventRecord(join_ev, side), "event");
         hip_check(hipStreamWaitEvent(st, join_ev, 0), "event");
@@ -1304,7 +1309,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         base.use_graph = 1;
         base.profile = 0;
         es_tune_result best{base.batch ? base.batch : 16, base.tile_h ? base.tile_h : 32, base.cv_bmax ? base.cv_bmax : 32,
-                            base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks};
+                            base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks, base.early_energies};
         const int ne = base.n_encodes;
         auto trial = [&](const es_tune_result &t) -> double {
             es_scorer_config c = base;
@@ -1313,6 +1318,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             c.cv_bmax = t.cv_bmax;
             c.cv_waves = t.cv_waves;
             c.cv_tile_blocks = t.cv_tile_blocks;
+            c.early_energies = t.early_energies;
             const int need = 3 * t.batch;
             if (nframes < need) return -1.0;  // too few frames for this batch
             BatchedScorer sc(c);
@@ -1341,7 +1347,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
                 es_tune_result t = keep;
                 set(t, v);
                 if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves &&
-                    t.cv_tile_blocks == keep.cv_tile_blocks)
+                    t.cv_tile_blocks == keep.cv_tile_blocks && t.early_energies == keep.early_energies)
                     continue;
                 const double ms = trial(t);
                 if (ms > 0 && ms < best.ms_per_step) {
@@ -1356,6 +1362,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         // with the energy chain overlapping, fewer resident c-value waves can leave it room
         // (0: one block per tile, which the hardware interleaves with the overlapping energy chain)
         try_set([](es_tune_result &t, int v) { t.cv_waves = v, t.cv_tile_blocks = v == 0; }, {0, 6, 8, 12, 16, 24});
+        if (!base.serial_stages) try_set([](es_tune_result &t, int v) { t.early_energies = v; }, {0, 1});
         try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32});
         const int trials = out->trials;
         *out = best;

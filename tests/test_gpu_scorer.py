@@ -213,7 +213,7 @@ def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
     r = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
     assert r["batch"] in (8, 16, 32) and r["tile_h"] in (16, 24, 32, 48, 64)
     assert r["cv_bmax"] in (32, 64) and r["cv_waves"] in (0, 6, 8, 12, 16, 24)
-    assert r["cv_tile_blocks"] == (r["cv_waves"] == 0)
+    assert r["cv_tile_blocks"] == (r["cv_waves"] == 0) and r["early_energies"] in (False, True)
     assert r["ms_per_step"] > 0 and r["trials"] >= 14 and not r["cached"]
     again = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
     assert again["cached"] and {k: again[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")} == \
@@ -230,9 +230,11 @@ def test_overlapped_and_serial_stages_agree(gpu):
     ds = [gpu.DeviceArray.from_host(f) for f in src]
     de = [gpu.DeviceArray.from_host(f) for f in enc]
     out = []
-    for serial, graph, profile in ((True, True, False), (False, False, False), (False, True, False), (False, True, True)):
+    for serial, graph, profile, early in ((True, True, False, False), (False, False, False, False),
+                                          (False, True, False, False), (False, True, True, False),
+                                          (False, True, False, True), (False, False, True, True)):
         sc = gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], batch=5, use_graph=graph,
-                        profile=profile, serial_stages=serial, per_segment=10)
+                        profile=profile, serial_stages=serial, per_segment=10, early_energies=early)
         sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
         sc.sync()
         out.append((sc.cambi(0, n), [sc.segment(0, s, with_hist=True) for s in range(sc.segments(0))]))
