@@ -127,6 +127,8 @@ def main():
     ap.add_argument("--cv-bmax", type=int, default=0, help="c-value LDS layout, 32 or 64 (with --tune 0)")
     ap.add_argument("--cv-waves", type=int, default=0, help="persistent c-value waves per CU (with --tune 0)")
     ap.add_argument("--cv-tile-blocks", type=int, default=0, help="c-value kernel as one block per tile (with --tune 0)")
+    ap.add_argument("--cv-mfma", type=int, default=0,
+                    help="c-value window counts on the matrix cores (with --tune 0)")
     ap.add_argument("--early-energies", type=int, default=0,
                     help="start the energy chain with the CAMBI pyramid rather than after it (with --tune 0)")
     ap.add_argument("--plain-steps", type=int, default=48)
@@ -228,7 +230,7 @@ def main():
     # the autotuner, on the pool's frames, outside the timed region
     tuned = None
     batch, tile_h, cv_bmax, cv_waves, cv_tb = args.batch, args.tile_h, args.cv_bmax, args.cv_waves, bool(args.cv_tile_blocks)
-    early = bool(args.early_energies)
+    early, mfma = bool(args.early_energies), bool(args.cv_mfma)
     if args.tune:
         n_tune = min(pool, 96)
         s_t = [src_frames[f].ptr for f in range(n_tune)]
@@ -238,13 +240,13 @@ def main():
                             serial_stages=bool(args.serial))
         tuned["seconds"] = round(time.perf_counter() - t0, 2)
         batch, tile_h, cv_bmax, cv_waves = tuned["batch"], tuned["tile_h"], tuned["cv_bmax"], tuned["cv_waves"]
-        cv_tb, early = tuned["cv_tile_blocks"], tuned["early_energies"]
+        cv_tb, early, mfma = tuned["cv_tile_blocks"], tuned["early_energies"], tuned["cv_mfma"]
         print(f"[bench] autotuner: {tuned}", file=sys.stderr, flush=True)
 
     # batched form (graph replay), with live per-stage events
     bat = es.Scorer(es.ES_BATCHED, W, H, descs, batch=batch, tile_h=tile_h, use_graph=True, profile=True,
                     src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves, serial_stages=bool(args.serial),
-                    cv_tile_blocks=cv_tb, early_energies=early)
+                    cv_tile_blocks=cv_tb, early_energies=early, cv_mfma=mfma)
     submit(bat, 0, args.warmup)
     bat.sync()
     warm_stage, warm_rounds = bat.stage_ms()
@@ -301,6 +303,7 @@ def main():
         "config": {"workload": workload, "config": args.config, "width": W, "height": H, "encodes": len(encs),
                    "batch": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32,
                    "cv_waves": "one block per tile" if cv_tb else (cv_waves or 16),
+                   "cv_method": "matrix cores" if mfma else "prefix scans",
                    "chroma": bool(args.chroma), "stages": "serial" if args.serial else
                    ("energies overlapped from the pyramid" if early else "energies overlapped from the c-values"),
                    "form": "batched, integer energies, graph replay"},

@@ -112,7 +112,8 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
     hip_check(hipMemsetAsync(counts_.p, 0, counts_.bytes, own_), "clearing counts");
     if (keep_) hip_check(hipMemsetAsync(cvals_.p, 0, cvals_.bytes, own_), "clearing c-values");
     for (int s = 1; s < kScales; s++) launch_cambi_scale(dpic, 1, scale_strips(info_.sw[s], info_.sh[s]), s, own_);
-    launch_cambi_cvalues(dpic, first_.as<int>(), kScales, tiles_, tile_h_, 0, nullptr, 0, 32, own_);
+    // one block per tile, window counts on the matrix cores
+    launch_cambi_cvalues(dpic, first_.as<int>(), kScales, tiles_, tile_h_, 0, nullptr, 0, 32, own_, 1);
     for (int pass = 0; pass < 3; pass++) launch_topk_pass(jobs_.as<TopkJob>(), kScales, pass, kTopkBlocks, own_);
     launch_cambi_finish(dpic, jobs_.as<TopkJob>(), 1, score_.as<double>(), own_);
     Results *r = results_.as<Results>();

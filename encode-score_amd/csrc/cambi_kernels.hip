@@ -462,6 +462,207 @@ __device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, unsigned long l
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------------
+// c-values on the matrix cores.  The same tiles, numbering and passes; per row the window count of every number of
+// the pass at every output column comes from one banded matrix product instead of prefix scans:
+//   W[x][n] = sum over margin columns c of A[x][c] C[c][n],   A[x][c] = [x <= c <= x + 2 pad],
+// with C the column counts (u8, at most 2 pad + 1) and W at most (2 pad + 1)^2, exact in i32.  The 64 output columns
+// are two blocks of 32 and the margin columns (64 + 2 pad <= 128) four blocks of 32; a block product is one
+// v_mfma_i32_32x32x32_i8 for 32 numbers, and only the blocks the band touches are multiplied (at most 3 per output
+// block).  A lane holds the band bytes of its output row and its 16 margin columns of each block, built once per
+// tile; the B operand is read 16 bytes at a time from a [number][column] byte table of the counts.  The products go
+// to a [number][column] u16 table from which each served pixel reads the counts of its numbers.  The lane layout of
+// the K dimension is the same for both operands (lane / 32 picks the 16-column half of the block), so the sum over
+// K does not depend on the hardware's order of it.
+// ---------------------------------------------------------------------------------------------------------------
+typedef int cv_v4i __attribute__((ext_vector_type(4)));
+typedef int cv_v16i __attribute__((ext_vector_type(16)));
+constexpr int kMCnt = 144;  // bytes per number in the count table: 128 columns, rows 16-byte aligned on spread banks
+constexpr int kMWin = 68;   // u16 per number in the window table: 64 columns and 4 of spread
+
+template <int NB>
+struct CvSharedM {
+    uint8_t cnt[32 * NB][kMCnt];  // column counts of number n at margin column c
+    uint16_t win[32 * NB][kMWin];  // window counts of number n at output column x
+    uint16_t rank[kMaxBand + 2 * kDiffs];
+    __device__ uint8_t *flags() { return (uint8_t *)&win[0][0]; }
+};
+static_assert(sizeof(uint16_t) * 32 * kMWin >= 2 * kCvFlags, "flags live in the window table during the setup");
+
+// The 16 band bytes [xr <= c <= xr + span] over margin columns c = cb .. cb + 15, one byte per column.
+__device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
+{
+    const int lo = min(max(xr - cb, 0), 16), hi = min(max(xr + span - cb + 1, 0), 16);
+    const unsigned m = ((1u << hi) - 1u) & ~((1u << lo) - 1u);  // bit t: column cb + t in the band
+    cv_v4i v;
+#pragma unroll
+    for (int d = 0; d < 4; d++) v[d] = (int)((((m >> (4 * d)) & 0xFu) * 0x00204081u) & 0x01010101u);  // bit i -> byte i
+    return v;
+}
+
+// Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds).
+template <int NB, int NBU>
+__device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
+{
+    static_assert(NBU >= 1 && NBU <= NB, "blocks of numbers");
+    constexpr int B = 32 * NBU;
+    const int lane = threadIdx.x, pad = t.pad, span = 2 * pad;
+    const int c0 = 2 * lane, xl = t.x0 - pad + c0;
+    const bool col0 = c0 < kCvTileW + 2 * pad, col1 = c0 + 1 < kCvTileW + 2 * pad;
+    const int x = t.x0 + lane;
+    const unsigned xo0 = col0 ? cv_col(t, xl) : kCvOut, xo1 = col1 ? cv_col(t, xl + 1) : kCvOut, xoo = cv_col(t, x);
+    // band operands of output block mb and margin block mb + j: this lane's row 32 mb + lane % 32 and its half of
+    // the block's columns; nj blocks per output block carry the band
+    const int g = lane >> 5, r32 = lane & 31, nj = min((31 + span) / 32 + 1, 3);
+    cv_v4i band[2][3];
+#pragma unroll
+    for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) band[mb][j] = band_bytes(32 * mb + r32, 32 * (mb + j) + 16 * g, span);
+    uint32_t *cnt_words = (uint32_t *)&sh.cnt[0][0];
+    for (int b0 = 0;;) {
+        const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
+        for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)&sh.cnt[0][0])[i] = cv_v4i{0, 0, 0, 0};
+        // A sample of rank rk entering (delta 1) or leaving (delta ~0) column c's count: an increment of zero when
+        // it is inactive or outside the pass, so that every lane makes the same LDS operation.  A lane's two columns
+        // share a word of every number's row.
+        auto bump = [&](unsigned rk, int c, unsigned delta) {
+            const unsigned b = rk - (unsigned)b0;
+            const unsigned inc = b < (unsigned)B ? delta << ((c & 3) << 3) : 0u;
+            atomicAdd(cnt_words + (b & (B - 1)) * (kMCnt / 4) + (c >> 2), inc);
+        };
+        const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
+        for (int y = ya; y < yb; y += 8) {
+            unsigned v[16];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int yy = y + j < yb ? y + j : -1;
+                v[2 * j] = cv_at(t, xo0, yy);
+                v[2 * j + 1] = cv_at(t, xo1, yy);
+            }
+            unsigned rk[16];
+#pragma unroll
+            for (int j = 0; j < 16; j++) rk[j] = sh.rank[cv_index(t, v[j])];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                bump(rk[2 * j], c0, 1u);
+                bump(rk[2 * j + 1], c0 + 1, 1u);
+            }
+        }
+        auto load = [&](int y, unsigned *v) {
+            v[0] = cv_at(t, xo0, y + pad);
+            v[1] = cv_at(t, xo1, y + pad);
+            v[2] = cv_at(t, xoo, y);
+            v[3] = cv_at(t, xo0, y - pad);
+            v[4] = cv_at(t, xo1, y - pad);
+        };
+        unsigned cur[5], nxt[5] = {0, 0, 0, 0, 0}, gone0 = 0, gone1 = 0;
+        load(t.y0, cur);
+        for (int r = 0; r < t.rows; r++) {
+            const int y = t.y0 + r;
+            if (r + 1 < t.rows) load(y + 1, nxt);
+            const unsigned io = cv_index(t, cur[2]), base = io ? io - kDiffs : 0u;
+            const unsigned re0 = sh.rank[cv_index(t, cur[0])], re1 = sh.rank[cv_index(t, cur[1])];
+            const unsigned rl0 = sh.rank[cv_index(t, gone0)], rl1 = sh.rank[cv_index(t, gone1)];
+            unsigned rk[2 * kDiffs + 1];
+#pragma unroll
+            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = sh.rank[base + k];
+            bump(rl0, c0, ~0u);
+            bump(rl1, c0 + 1, ~0u);
+            bump(re0, c0, 1u);
+            bump(re1, c0 + 1, 1u);
+            const unsigned own = io ? rk[kDiffs] : (unsigned)kNone;
+            const bool serve = own != kNone && (int)own >= lo && (int)own < hi;
+            if (__builtin_amdgcn_ballot_w64(serve)) {
+                // the LDS operations of one wave complete in order: the row's updates are in before these reads
+                asm volatile("" ::: "memory");
+                cv_v4i bop[NBU][4];
+#pragma unroll
+                for (int nb = 0; nb < NBU; nb++)
+#pragma unroll
+                    for (int kb = 0; kb < 4; kb++)
+                        bop[nb][kb] = kb <= nj ? *(const cv_v4i *)&sh.cnt[32 * nb + r32][32 * kb + 16 * g] : cv_v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+                    for (int nb = 0; nb < NBU; nb++) {
+                        cv_v16i acc = {};
+#pragma unroll
+                        for (int j = 0; j < 3; j++)
+                            if (j < nj) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc, 0, 0, 0);
+                        // D's register 4 q + i of this lane: output column 32 mb + 8 q + 4 (lane / 32) + i, number
+                        // 32 nb + lane % 32; four columns per 8-byte store
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            const uint2 w2 = make_uint2((unsigned)acc[4 * q] | (unsigned)acc[4 * q + 1] << 16,
+                                                        (unsigned)acc[4 * q + 2] | (unsigned)acc[4 * q + 3] << 16);
+                            *(uint2 *)&sh.win[32 * nb + r32][32 * mb + 8 * q + 4 * g] = w2;
+                        }
+                    }
+                asm volatile("" ::: "memory");
+                unsigned cnt[2 * kDiffs + 1];
+#pragma unroll
+                for (int k = 0; k < 2 * kDiffs + 1; k++) {
+                    const unsigned b = rk[k] - (unsigned)b0, bb = b & (B - 1);
+                    const unsigned d = sh.win[bb][lane];
+                    cnt[k] = b < (unsigned)B ? d : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 2 * kDiffs + 1; k++) asm volatile("" ::"v"(cnt[k]));
+                const int value = (int)(io ? io - kDiffs : 0u) + t.band_base + kDiffs;
+                const unsigned p0 = cnt[kDiffs];
+                float c = 0.0f;
+#pragma unroll
+                for (int d = 0; d < kDiffs; d++) {
+                    const unsigned pa = cnt[kDiffs + d + 1], pb = cnt[kDiffs - d - 1];
+                    const unsigned pm = pa > pb ? pa : pb;
+                    const float rcp = recip_rn(p0 + pm);
+                    const float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
+                    const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt && pm > 0;
+                    c = on && val > c ? val : c;
+                }
+                c = serve ? c : 0.0f;
+                if (t.plane) {
+                    if (serve) t.plane[(unsigned)(y * t.w + x)] = c;
+                }
+                sum_fx += c > 0.0f ? fixed24(c) : 0ull;
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
+                if (c > 0.0f) t.seg[tcount + __popcll(m & ((1ull << lane) - 1))] = c;
+                tcount += (unsigned)__popcll(m);
+                asm volatile("" ::: "memory");  // every read of the table before the next row writes it
+            }
+            gone0 = cur[3];
+            gone1 = cur[4];
+#pragma unroll
+            for (int k = 0; k < 5; k++) cur[k] = nxt[k];
+        }
+        if (hi == relevant) break;
+        b0 = hi - kDiffs;
+    }
+}
+
+template <int BMAX>
+__device__ __forceinline__ uint8_t *cv_flags(CvShared<BMAX> &sh) { return (uint8_t *)&sh.pp[0][0]; }
+template <int NB>
+__device__ __forceinline__ uint8_t *cv_flags(CvSharedM<NB> &sh) { return sh.flags(); }
+
+template <int BMAX>
+__device__ __forceinline__ void cv_run(CvShared<BMAX> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
+                                       unsigned &tcount)
+{
+    if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_fx, tcount);
+    else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_fx, tcount);
+    else if (BMAX == 32 || relevant <= 32) cv_passes<32>(sh, t, relevant, sum_fx, tcount);
+    else cv_passes<(BMAX > 32 ? BMAX : 32)>(sh, t, relevant, sum_fx, tcount);
+}
+template <int NB>
+__device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
+                                       unsigned &tcount)
+{
+    if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1>(sh, t, relevant, sum_fx, tcount);
+    else cv_passes_m<NB, (NB > 1 ? 2 : 1)>(sh, t, relevant, sum_fx, tcount);
+}
+
 __device__ int find_slot(const int *first, int n, int t)
 {
     int lo = 0, hi = n - 1;
@@ -473,8 +674,8 @@ __device__ int find_slot(const int *first, int n, int t)
     return lo;
 }
 
-template <int BMAX>
-__device__ void cv_tile(CvShared<BMAX> &sh, const CambiPic &pic, int s, int local, int tile_h)
+template <typename Sh>
+__device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_h)
 {
     const CambiConsts &K = *pic.consts;
     CvTile t;
@@ -507,7 +708,7 @@ __device__ void cv_tile(CvShared<BMAX> &sh, const CambiPic &pic, int s, int loca
     const int lane = threadIdx.x, nv = t.band_size + 2 * kDiffs;
     const int c0 = 2 * lane, xl = t.x0 - t.pad + c0, cw = kCvTileW + 2 * t.pad;
     __syncthreads();  // the previous tile's reads of pp and rank are done
-    uint8_t *served = (uint8_t *)&sh.pp[0][0], *occupied = served + kCvFlags;
+    uint8_t *served = cv_flags(sh), *occupied = served + kCvFlags;
     for (int i = lane; i < 2 * kCvFlags / 4; i += 64) ((uint32_t *)served)[i] = 0;
     __syncthreads();
     bool any = false;
@@ -556,12 +757,7 @@ __device__ void cv_tile(CvShared<BMAX> &sh, const CambiPic &pic, int s, int loca
     __syncthreads();
     unsigned long long sum_fx = 0;
     unsigned tcount = 0;
-    if (relevant > 0) {
-        if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_fx, tcount);
-        else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_fx, tcount);
-        else if (BMAX == 32 || relevant <= 32) cv_passes<32>(sh, t, relevant, sum_fx, tcount);
-        else cv_passes<(BMAX > 32 ? BMAX : 32)>(sh, t, relevant, sum_fx, tcount);
-    }
+    if (relevant > 0) cv_run(sh, t, relevant, sum_fx, tcount);
     sum_fx = wave_sum(sum_fx);
     if (lane == 0) {
         *seg_count = tcount;
@@ -570,11 +766,12 @@ __device__ void cv_tile(CvShared<BMAX> &sh, const CambiPic &pic, int s, int loca
     }
 }
 
-template <int BMAX>
+// Sh: CvShared<BMAX> (prefix scans on the vector units) or CvSharedM<NB> (window counts on the matrix cores).
+template <typename Sh>
 __global__ __launch_bounds__(64) void cambi_cvalues_kernel(const CambiPic *pics, const int *tile_first, int nslots,
                                                            int tile_h, int persistent, unsigned int *counter)
 {
-    __shared__ CvShared<BMAX> sh;
+    __shared__ Sh sh;
     const int total = tile_first[nslots];
     for (int t = persistent ? 0 : (int)blockIdx.x;;) {
         if (persistent) {
@@ -591,16 +788,13 @@ __global__ __launch_bounds__(64) void cambi_cvalues_kernel(const CambiPic *pics,
 }
 
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st)
+                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma)
 {
     if (!persistent && total_tiles <= 0) return;
     const int grid = persistent ? blocks : total_tiles;
-    if (bmax == 64)
-        hipLaunchKernelGGL(cambi_cvalues_kernel<64>, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h,
-                           persistent, work_counter);
-    else
-        hipLaunchKernelGGL(cambi_cvalues_kernel<32>, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h,
-                           persistent, work_counter);
+    auto k = mfma ? (bmax == 64 ? cambi_cvalues_kernel<CvSharedM<2>> : cambi_cvalues_kernel<CvSharedM<1>>)
+                  : (bmax == 64 ? cambi_cvalues_kernel<CvShared<64>> : cambi_cvalues_kernel<CvShared<32>>);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h, persistent, work_counter);
     hip_check(hipGetLastError(), "cambi c-values");
 }
 

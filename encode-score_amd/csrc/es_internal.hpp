@@ -98,8 +98,9 @@ void launch_cambi_base(const CambiPic *pics, const int *tile_first, int npics, i
 int scale_strips(int w, int h);
 void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, hipStream_t st);
 // tile_first[nslots] (device) holds the total; total_tiles sizes the grid when not persistent.
+// mfma: window counts on the matrix cores instead of prefix scans (same results).
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st);
+                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma = 0);
 void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks, hipStream_t st);
 void launch_cambi_finish(const CambiPic *pics, TopkJob *jobs, int npics, double *scores, hipStream_t st);
 void launch_clear(unsigned long long *a, long long na, unsigned int *b, long long nb, hipStream_t st);

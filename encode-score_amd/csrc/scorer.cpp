@@ -450,6 +450,7 @@ public:
         require(c.cv_bmax == 0 || c.cv_bmax == 32 || c.cv_bmax == 64, "cv_bmax must be 32 or 64");
         cv_bmax = c.cv_bmax ? c.cv_bmax : 32;
         cv_tile_blocks = c.cv_tile_blocks != 0;
+        cv_mfma = c.cv_mfma != 0;
         const int C = (int)crops.size(), E = (int)enc.size();
         roles = C + E;
         P = B * roles;
@@ -682,7 +683,7 @@ private:
     };
 
     int B, R, P, roles, tile_h, cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
-    bool cv_tile_blocks = false;
+    bool cv_tile_blocks = false, cv_mfma = false;
     std::vector<long long> max_px;
     std::vector<int> max_strips;
     struct Layout {
@@ -871,7 +872,7 @@ private:
             // persistent waves taking tiles from a counter, or one block per tile of the round's largest layout (the
             // blocks past the round's tiles exit at once)
             launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, (int)seg_off[P], tile_h, cv_tile_blocks ? 0 : 1,
-                                 counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st);
+                                 counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st, cv_mfma ? 1 : 0);
             break;
         case 4:
             for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, kTopkBlocks, st);
@@ -1309,7 +1310,8 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         base.use_graph = 1;
         base.profile = 0;
         es_tune_result best{base.batch ? base.batch : 16, base.tile_h ? base.tile_h : 32, base.cv_bmax ? base.cv_bmax : 32,
-                            base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks, base.early_energies};
+                            base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks, base.early_energies,
+                            base.cv_mfma};
         const int ne = base.n_encodes;
         auto trial = [&](const es_tune_result &t) -> double {
             es_scorer_config c = base;
@@ -1319,6 +1321,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             c.cv_waves = t.cv_waves;
             c.cv_tile_blocks = t.cv_tile_blocks;
             c.early_energies = t.early_energies;
+            c.cv_mfma = t.cv_mfma;
             const int need = 3 * t.batch;
             if (nframes < need) return -1.0;  // too few frames for this batch
             BatchedScorer sc(c);
@@ -1347,7 +1350,8 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
                 es_tune_result t = keep;
                 set(t, v);
                 if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves &&
-                    t.cv_tile_blocks == keep.cv_tile_blocks && t.early_energies == keep.early_energies)
+                    t.cv_tile_blocks == keep.cv_tile_blocks && t.early_energies == keep.early_energies &&
+                    t.cv_mfma == keep.cv_mfma)
                     continue;
                 const double ms = trial(t);
                 if (ms > 0 && ms < best.ms_per_step) {
@@ -1356,6 +1360,8 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
                 }
             }
         };
+        // window counts on the matrix cores or by prefix scans
+        try_set([](es_tune_result &t, int v) { t.cv_mfma = v; }, {0, 1});
         // tile height and LDS layout together: taller tiles see more values, which favours the wider passes
         try_set([](es_tune_result &t, int v) { t.tile_h = v % 100, t.cv_bmax = v / 100; },
                 {3216, 3224, 3232, 3248, 3264, 6416, 6424, 6432, 6448, 6464});

@@ -179,20 +179,21 @@ def test_batched_larger_windows_match_oracle(gpu, orc, win):
     assert np.array_equal(hist, ref[1])
 
 
+@pytest.mark.parametrize("mfma", [False, True])
 @pytest.mark.parametrize("bmax,tile_h,waves", [(64, 32, 16), (64, 16, 8), (32, 48, 24), (32, 64, 12), (32, 32, 0),
                                                (64, 24, 0)])
-def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves):
+def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves, mfma):
     """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, waves per CU or
     one block per tile (waves 0)) gives the oracle's results; textured 640x360 frames have tiles with more than 32
     relevant values."""
     n, w, h = 14, 640, 360
-    print(f"case layout bmax {bmax} tile_h {tile_h} waves {waves}", flush=True)
+    print(f"case layout bmax {bmax} tile_h {tile_h} waves {waves} mfma {mfma}", flush=True)
     src, enc = frames_of(gpu, n, w=w, h=h, seed=41, full=(w, h))
     run, cam, cam_src = oracle_run(orc, src, enc, 1440)
     ds = [gpu.DeviceArray.from_host(f) for f in src]
     de = [gpu.DeviceArray.from_host(f) for f in enc]
     sc = gpu.Scorer(gpu.ES_BATCHED, w, h, [gpu.EncodeDesc(w, h, n, w, 2, 0)], batch=8, use_graph=True, tile_h=tile_h,
-                    cv_bmax=bmax, cv_waves=waves, cv_tile_blocks=waves == 0)
+                    cv_bmax=bmax, cv_waves=waves, cv_tile_blocks=waves == 0, cv_mfma=mfma)
     sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
     sc.sync()
     a, b = sc.cambi(0, n)
