@@ -513,7 +513,8 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
     const unsigned xo0 = col0 ? cv_col(t, xl) : kCvOut, xo1 = col1 ? cv_col(t, xl + 1) : kCvOut, xoo = cv_col(t, x);
     // band operands of output block mb and margin block mb + j: this lane's row 32 mb + lane % 32 and its half of
     // the block's columns; nj blocks per output block carry the band
-    const int g = lane >> 5, r32 = lane & 31, nj = min((31 + span) / 32 + 1, 3);
+    const int g = lane >> 5, r32 = lane & 31;
+    const int nj = __builtin_amdgcn_readfirstlane(min((31 + span) / 32 + 1, 3));  // the same on every lane
     cv_v4i band[2][3];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++)
@@ -583,22 +584,26 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                     for (int kb = 0; kb < 4; kb++)
                         bop[nb][kb] = kb <= nj ? *(const cv_v4i *)&sh.cnt[32 * nb + r32][32 * kb + 16 * g] : cv_v4i{0, 0, 0, 0};
 #pragma unroll
-                for (int mb = 0; mb < 2; mb++)
+                for (int nb = 0; nb < NBU; nb++) {
+                    // the two output blocks' products interleaved, so that neither waits on its own previous one
+                    cv_v16i acc[2] = {};
 #pragma unroll
-                    for (int nb = 0; nb < NBU; nb++) {
-                        cv_v16i acc = {};
+                    for (int j = 0; j < 3; j++)
+                        if (j < nj)
 #pragma unroll
-                        for (int j = 0; j < 3; j++)
-                            if (j < nj) acc = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc, 0, 0, 0);
-                        // D's register 4 q + i of this lane: output column 32 mb + 8 q + 4 (lane / 32) + i, number
-                        // 32 nb + lane % 32; four columns per 8-byte store
+                            for (int mb = 0; mb < 2; mb++)
+                                acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc[mb], 0, 0, 0);
+                    // D's register 4 q + i of this lane: output column 32 mb + 8 q + 4 (lane / 32) + i, number
+                    // 32 nb + lane % 32; four columns per 8-byte store
+#pragma unroll
+                    for (int mb = 0; mb < 2; mb++)
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
-                            const uint2 w2 = make_uint2((unsigned)acc[4 * q] | (unsigned)acc[4 * q + 1] << 16,
-                                                        (unsigned)acc[4 * q + 2] | (unsigned)acc[4 * q + 3] << 16);
+                            const uint2 w2 = make_uint2((unsigned)acc[mb][4 * q] | (unsigned)acc[mb][4 * q + 1] << 16,
+                                                        (unsigned)acc[mb][4 * q + 2] | (unsigned)acc[mb][4 * q + 3] << 16);
                             *(uint2 *)&sh.win[32 * nb + r32][32 * mb + 8 * q + 4 * g] = w2;
                         }
-                    }
+                }
                 asm volatile("" ::: "memory");
                 unsigned cnt[2 * kDiffs + 1];
 #pragma unroll
@@ -768,8 +773,8 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
 
 // Sh: CvShared<BMAX> (prefix scans on the vector units) or CvSharedM<NB> (window counts on the matrix cores).
 template <typename Sh>
-__global__ __launch_bounds__(64) void cambi_cvalues_kernel(const CambiPic *pics, const int *tile_first, int nslots,
-                                                           int tile_h, int persistent, unsigned int *counter)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void cambi_cvalues_kernel(
+    const CambiPic *pics, const int *tile_first, int nslots, int tile_h, int persistent, unsigned int *counter)
 {
     __shared__ Sh sh;
     const int total = tile_first[nslots];
