@@ -58,10 +58,10 @@ int base_tiles(int w, int h) { return ((w + kBaseW - 1) / kBaseW) * ((h + kBaseR
 // inside is applied where the value is used (outside, it reads as -1), so that the loads of row r + 2 issued during
 // row r are first waited for during row r + 1.
 template <typename Read>
-__device__ __forceinline__ void base_strip(const CambiPic &pic, Read read)
+__device__ __forceinline__ void base_strip(const CambiPic &pic, Read read, unsigned bx)
 {
     const int w = pic.sw[0], h = pic.sh[0];
-    const int lane = threadIdx.x & 63, strip = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, strip = (int)bx * 4 + (threadIdx.x >> 6);
     const int across = (w + kBaseW - 1) / kBaseW;
     if (strip >= across * ((h + kBaseRows - 1) / kBaseRows)) return;  // uniform over the wave
     const int x0 = (strip % across) * kBaseW, y0 = (strip / across) * kBaseRows;
@@ -125,20 +125,21 @@ __device__ __forceinline__ void base_strip(const CambiPic &pic, Read read)
 
 __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
 {
-    const CambiPic &pic = pics[blockIdx.y];
+    const uint3 b = xcd_block3();  // neighbouring strips share halo columns and rows
+    const CambiPic &pic = pics[b.y];
     const int w = pic.sw[0], h = pic.sh[0];
     const PlaneIn in = pic.in;
     auto cx = [=](int xx) { return min(max(xx, 0), w - 1); };
     auto cy = [=](int yy) { return (size_t)min(max(yy, 0), h - 1) * in.pitch; };
     // one reader per plane format, chosen once per block
     if (in.from8) {
-        base_strip(pic, [=](int xx, int yy) { return (int)base_value(in, w, h, cx(xx), min(max(yy, 0), h - 1)); });
+        base_strip(pic, [=](int xx, int yy) { return (int)base_value(in, w, h, cx(xx), min(max(yy, 0), h - 1)); }, b.x);
     } else if (in.bytes == 1) {
         const ES_GLOBAL uint8_t *p = (const ES_GLOBAL uint8_t *)in.ptr;
-        base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); });
+        base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); }, b.x);
     } else {
         const ES_GLOBAL uint16_t *p = (const ES_GLOBAL uint16_t *)in.ptr;
-        base_strip(pic, [=](int xx, int yy) { return (int)(((unsigned)p[cy(yy) + cx(xx)] >> in.rshift) << in.shift); });
+        base_strip(pic, [=](int xx, int yy) { return (int)(((unsigned)p[cy(yy) + cx(xx)] >> in.rshift) << in.shift); }, b.x);
     }
 }
 
@@ -161,9 +162,10 @@ int scale_strips(int w, int h) { return ((w + 63) / 64) * ((h + kScaleRows - 1) 
 
 __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, int s)
 {
-    const CambiPic &pic = pics[blockIdx.y];
+    const uint3 b = xcd_block3();  // neighbouring strips share halo rows
+    const CambiPic &pic = pics[b.y];
     const int w = pic.sw[s], h = pic.sh[s], pw = pic.sw[s - 1];
-    const int lane = threadIdx.x & 63, strip = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, strip = (int)b.x * 4 + (threadIdx.x >> 6);
     const int across = (w + 63) / 64;
     if (strip >= across * ((h + kScaleRows - 1) / kScaleRows)) return;  // uniform over the wave
     const int x0 = (strip % across) * 64, y0 = (strip / across) * kScaleRows, y1 = min(y0 + kScaleRows, h);

@@ -8,6 +8,19 @@
 
 namespace es {
 
+// The hardware hands a launch's workgroups to the 8 XCDs in turn (workgroup i, counting x fastest, to XCD i % 8), and
+// each XCD has its own L2.  For tiles that share halo rows, halo columns or cache lines with their neighbours, this
+// renumbers the workgroups so that each XCD takes one contiguous run of them (in x-fastest order): neighbours then
+// meet in the same L2 instead of fetching the shared lines from HBM once per XCD.
+__device__ __forceinline__ uint3 xcd_block3()
+{
+    const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+    const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned xcd = lin % 8u, k = lin / 8u, q = n / 8u, r = n % 8u;
+    const unsigned l = xcd * q + min(xcd, r) + k;  // XCD x takes q (+1 for the first r) consecutive numbers
+    return make_uint3(l % gx, (l / gx) % gy, l / (gx * gy));
+}
+
 __device__ __forceinline__ unsigned raw_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;

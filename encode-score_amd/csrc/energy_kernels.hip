@@ -182,12 +182,12 @@ typedef int WsRows[2][2][72 + 64];  // [buffer][picture][column - (x0 - 4)]; 72 
 
 // One strip of the pair; BR and BD are the source's and the encode's bytes per sample.
 template <int WIN, int BR, int BD>
-__device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf)
+__device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsigned bx, unsigned by)
 {
     static_assert(kWsStrip % WIN == 0 && WIN % 16 == 0, "windows tile the strip in 16-row blocks");
     const int lane = threadIdx.x;
     const int w = P.w, h = P.h, ww = w / WIN, H = h / WIN * WIN, W = ww * WIN;
-    const int x0 = blockIdx.x * 64, y0 = blockIdx.y * kWsStrip;
+    const int x0 = (int)bx * 64, y0 = (int)by * kWsStrip;
     if (x0 >= W || y0 >= H) return;  // uniform over the block
     const int y1 = min(y0 + kWsStrip, H), rend = y1 + 3;  // rows r = y0 - 4 .. rend
     const int x = x0 + lane;
@@ -318,14 +318,15 @@ template <int WIN>
 __global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pairs)
 {
     __shared__ WsRows rowbuf;
-    const WsPair P = pairs[blockIdx.z];  // a copy: the stores cannot change it, so it is read once
+    const uint3 b = xcd_block3();
+    const WsPair P = pairs[b.z];  // a copy: the stores cannot change it, so it is read once
     // one reader per pair of plane formats, chosen once per block
     if (P.r.bytes == 1) {
-        if (P.d.bytes == 1) ws_strip<WIN, 1, 1>(P, rowbuf);
-        else ws_strip<WIN, 1, 2>(P, rowbuf);
+        if (P.d.bytes == 1) ws_strip<WIN, 1, 1>(P, rowbuf, b.x, b.y);
+        else ws_strip<WIN, 1, 2>(P, rowbuf, b.x, b.y);
     } else {
-        if (P.d.bytes == 1) ws_strip<WIN, 2, 1>(P, rowbuf);
-        else ws_strip<WIN, 2, 2>(P, rowbuf);
+        if (P.d.bytes == 1) ws_strip<WIN, 2, 1>(P, rowbuf, b.x, b.y);
+        else ws_strip<WIN, 2, 2>(P, rowbuf, b.x, b.y);
     }
 }
 
@@ -403,10 +404,11 @@ __global__ __launch_bounds__(256) void persist_plain_kernel(const PersistJob *jo
 template <int WIN>
 __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
 {
-    const LastingJob &J = jobs[blockIdx.z];
+    const uint3 b = xcd_block3();  // neighbouring patches share cache lines
+    const LastingJob &J = jobs[b.z];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ww = J.w / WIN, wh = J.h / WIN, pw = ww * (WIN / 16), ph = wh * (WIN / 16);
-    const int patch = blockIdx.x * 4 + wave;
+    const int patch = (int)b.x * 4 + wave;
     if (J.ncentres <= 0 || patch >= pw * ph) return;  // uniform over the wave
     const int px = patch % pw, py = patch / pw;
     const int x = px * 16 + (lane & 3) * 4, y = py * 16 + (lane >> 2);
