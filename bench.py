@@ -73,18 +73,19 @@ def config(n, es, frames):
     raise SystemExit(f"no configuration {n}")
 
 
-STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{bmax}>", "cambi_base": "cambi_base_kernel",
+STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{layout}>", "cambi_base": "cambi_base_kernel",
                 "window_stats": "void window_stats_int_kernel<16>", "persist": "void lasting_kernel<16>",
                 "frame_sums": "frame_sums_kernel"}
 TRAFFIC_FILES = {(2, 3840, 2160): "hbm_traffic_3840x2160_mosaic.json"}
 
 
-def measured_traffic(cfg, w, h, batch, stage, bmax):
+def measured_traffic(cfg, w, h, batch, stage, bmax, mfma):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of this
     workload (scaled by the copy-kernel calibration recorded beside them, and by batch / the profiled batch: every
     kernel's traffic is per frame of the launch), or (None, reason)."""
     name = TRAFFIC_FILES.get((cfg, w, h))
-    kernel = STAGE_KERNEL[stage].format(bmax=bmax)
+    layout = f"CvSharedM<{1 if bmax == 32 else 2}> " if mfma else f"CvShared<{bmax}> "
+    kernel = STAGE_KERNEL[stage].format(layout=layout)
     if name is None:
         return None, "no FETCH_SIZE/WRITE_SIZE profile of this workload"
     path = os.path.join(ROOT, "profiles", name)
@@ -282,7 +283,7 @@ def main():
     dominant = max(alg_bytes, key=lambda k: stage[k])
     mean_ms = stage[dominant] / launches
     achieved = alg_bytes[dominant] / (mean_ms / 1e3) / 1e9
-    traffic, traffic_src = measured_traffic(args.config, W, H, batch, dominant, cv_bmax or 32)
+    traffic, traffic_src = measured_traffic(args.config, W, H, batch, dominant, cv_bmax or 32, mfma)
     # the whole step's input bytes (each luma plane, and with chroma each chroma plane, read once) against HBM
     b_in = (sbytes * n_src + sum(2 * d.w * d.h for d, _ in encs)) * (1.5 if args.chroma else 1.0)
     step_gbs = b_in * steps_per_s / 1e9
