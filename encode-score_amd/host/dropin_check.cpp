@@ -1,5 +1,8 @@
 // Compile check and GPU demo of the drop-in headers: the reference's calls (Kernels::load, Cambi(k, w, h, depth),
-// launch(img, pitch, stream), finish()) against libencscore_hip.  Prints the CAMBI score of a synthetic picture.
+// launch(img, pitch, stream), finish(); CambiCache(path, x0, y0, w, h, depth), find, add) against libencscore_hip.
+// Prints the CAMBI score of a synthetic picture; with a second argument (a file), also keeps it in that file's
+// source-CAMBI cache under timestamp 0 and prints what a second cache of the same file finds.
+#include "cache.hpp"
 #include "cambi.hpp"
 
 #include <cstdio>
@@ -27,7 +30,17 @@ int main(int argc, char **argv)
         es_check(es_synth_frame(&d, 0, img, 2, stream), "generating");
         Cambi cambi(k, w, h, 10);
         cambi.launch(img, w, stream);
-        std::printf("%.17g\n", cambi.finish());
+        const double score = cambi.finish();
+        std::printf("%.17g\n", score);
+        if (argc > 2) {
+            {
+                CambiCache cache(argv[2], 0, 0, w, h, 10);
+                cache.add(0, score);
+            }
+            CambiCache again(argv[2], 0, 0, w, h, 10);
+            const std::optional<double> found = again.find(0);
+            std::printf("cached %.17g in %s\n", found ? *found : -1.0, again.file().c_str());
+        }
         es_free(img);
         es_stream_destroy(stream);
     } catch (const std::exception &e) {

@@ -64,7 +64,7 @@ def test_synth_is_deterministic_and_in_range(es):
 
 
 def test_dropin_headers_compile():
-    """The C++ drop-in headers (gpu.hpp / cambi.hpp over the C ABI) compile with the host compiler alone."""
+    """The C++ drop-in headers (gpu.hpp / cambi.hpp / cache.hpp over the C ABI) compile with the host compiler alone."""
     import subprocess
     src = os.path.join(ROOT, "encode-score_amd", "host", "dropin_check.cpp")
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
@@ -110,3 +110,46 @@ def test_frame_arguments_are_checked_before_any_device_work(es):
         es.frame_planes(f8, 0, 0, 10, 0x4002, 0x8000, 0xC000, 64, 32)
     with pytest.raises(es.EsError):
         es.color_sums(0x1000, 0x2000, 10, 3, 0x3000)
+
+
+def test_dropin_cambi_cache_class(orc, tmp_path):
+    """The C++ drop-in CambiCache (host/cache.hpp, src/cache.hpp's interface) on the host: a score kept by one
+    instance is found by the next, in the file the oracle's restatement of the naming names."""
+    import subprocess
+    src = tmp_path / "cache_check.cpp"
+    src.write_text(r'''
+#include "cache.hpp"
+#include <cstdio>
+int main(int argc, char **argv)
+{
+    try {
+        {
+            CambiCache c(argv[1], 0, 140, 1920, 800, 10);
+            if (c.find(1001)) return 2;
+            c.add(1001, 3.25);
+            c.add(2002, 4.5);
+        }
+        CambiCache again(argv[1], 0, 140, 1920, 800, 10);
+        const auto a = again.find(1001), b = again.find(2002), n = again.find(3003);
+        std::printf("%s %g %g %d\n", again.file().c_str(), a ? *a : -1.0, b ? *b : -1.0, n ? 1 : 0);
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 1;
+    }
+    return 0;
+}
+''')
+    exe = tmp_path / "cache_check"
+    lib_dir = os.path.join(ROOT, "encode-score_amd")
+    r = subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(lib_dir, "host"),
+                        str(src), "-L", lib_dir, "-lencscore_hip", "-Wl,-rpath," + lib_dir, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    source = tmp_path / "film.mkv"
+    source.write_bytes(b"frames" * 5000)
+    env = dict(os.environ, XDG_CACHE_HOME=str(tmp_path / "xdg"))
+    r = subprocess.run([str(exe), str(source)], capture_output=True, text=True, env=env)
+    assert r.returncode == 0, r.stderr
+    path, a, b, n = r.stdout.split()
+    assert path == str(tmp_path / "xdg" / "encode-score" / orc.cambi_cache_name(source, 355, 0, 140, 1920, 800, 10))
+    assert (float(a), float(b), n) == (3.25, 4.5, "0")
