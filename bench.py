@@ -145,6 +145,8 @@ def main():
                     help="run each round's CAMBI and energy chains one after the other (1) instead of overlapped (0)")
     ap.add_argument("--chroma", type=int, default=1,
                     help="frames carry chroma planes: PSNR and mean colours in every step (1) or luma only (0)")
+    ap.add_argument("--cache-run", type=int, default=1,
+                    help="also time a second run whose source CAMBI comes from the source-CAMBI cache")
     ap.add_argument("--ingest", type=int, default=50,
                     help="time this many launches of the frame-ingest kernel on a P010 frame of the source's size")
     args = ap.parse_args()
@@ -208,9 +210,16 @@ def main():
              for f in range(f0, f0 + n)]
         return s, e
 
-    def submit(scorer, f0, n, stream=None):
-        if args.chroma:
-            scorer.submit_frames(*yuv(f0, n), stream)
+    def submit(scorer, f0, n, stream=None, pts=None):
+        if args.chroma or pts is not None:
+            if args.chroma:
+                s_, e_ = yuv(f0, n)
+            else:
+                s_ = [es.frame_view("yuv420p10" if sbytes == 2 else "yuv420p", src_frames[f % pool], None, None,
+                                    W * sbytes, W) for f in range(f0, f0 + n)]
+                e_ = [[es.frame_view("yuv420p10", enc_frames[i][f % pool], None, None, encs[i][0].w * 2, encs[i][0].w)
+                       for i in range(len(encs))] for f in range(f0, f0 + n)]
+            scorer.submit_frames(s_, e_, stream, pts=None if pts is None else pts[f0:f0 + n])
         else:
             scorer.submit(*ptrs(f0, n), stream)
 
@@ -318,6 +327,38 @@ def main():
                      "traffic_source": traffic_src,
                      "mean_launch_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes[dominant])},
     }
+
+    # the source-CAMBI cache (src/cache.cpp): a first run against the source fills it, a second run reads every
+    # source CAMBI from it and skips computing it; the second run is timed like the first
+    if args.cache_run:
+        import tempfile
+        with tempfile.TemporaryDirectory() as tmp:
+            source = os.path.join(tmp, "source")
+            with open(source, "wb") as fh:
+                fh.write(b"synthetic source" * 4096)
+            pts = np.arange(total, dtype=np.int64) * 1001
+            for k in range(2):
+                sc = es.Scorer(es.ES_BATCHED, W, H, descs, batch=batch, tile_h=tile_h, use_graph=True,
+                               src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves,
+                               serial_stages=bool(args.serial), cv_tile_blocks=cv_tb, early_energies=early, cv_mfma=mfma)
+                sc.use_cache(source, tmp)
+                submit(sc, 0, args.warmup, pts=pts)
+                sc.sync()
+                es.synchronize()
+                t0 = time.perf_counter()
+                submit(sc, args.warmup, args.steps, pts=pts)
+                sc.sync()
+                es.synchronize()
+                cwall = time.perf_counter() - t0
+                hits = sc.cache_hits()
+                cam_cached = sc.cambi(0, min(descs[0].length, total))
+                del sc
+            ref_cam = bat.cambi(0, min(descs[0].length, total))
+            result["source_cambi_cached"] = {
+                "value": round(args.steps / cwall, 3), "unit": "frame steps/s", "hits": hits,
+                "speedup": round(wall / cwall, 3),
+                "equal_figures": bool(np.array_equal(cam_cached[0], ref_cam[0]) and np.array_equal(cam_cached[1], ref_cam[1]))}
+            print(f"[bench] source CAMBI cached: {args.steps / cwall:.1f} steps/s", file=sys.stderr, flush=True)
 
     # plain per-frame-launch FP64 form on the same frames
     if not args.no_plain:
