@@ -31,11 +31,14 @@ __device__ __forceinline__ unsigned base_value(const PlaneIn &in, int w, int h, 
 
 // Neighbour lanes across the whole wave by DPP (lane i takes lane i + 1, or lane i - 1); the lane with no neighbour
 // gets 0 and is replaced by its caller.
-__device__ __forceinline__ int from_next_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
-__device__ __forceinline__ int from_prev_lane(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int from_next_lane(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ int from_prev_lane(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xF, 0xF, true); }
+// The same shifts with the lane that has no neighbour (63, or 0) taking `edge` instead.
+__device__ __forceinline__ int from_next_lane_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x130, 0xF, 0xF, false); }
+__device__ __forceinline__ int from_prev_lane_or(int v, int edge) { return __builtin_amdgcn_update_dpp(edge, v, 0x138, 0xF, 0xF, false); }
 
-// mode3 (src/kernels.cu:55-60)
-__device__ __forceinline__ uint16_t mode3(uint16_t a, uint16_t b, uint16_t c)
+// mode3 (src/kernels.cu:55-60) of 16-bit values held in 32-bit registers
+__device__ __forceinline__ unsigned mode3(unsigned a, unsigned b, unsigned c)
 {
     if (a == b || a == c) return a;
     if (b == c) return b;
@@ -54,73 +57,175 @@ constexpr int kBaseW = 64, kBaseRows = 64;
 
 int base_tiles(int w, int h) { return ((w + kBaseW - 1) / kBaseW) * ((h + kBaseRows - 1) / kBaseRows); }
 
-// One strip.  read(xx, yy) loads the sample at the nearest position inside the picture; whether (xx, yy) itself is
-// inside is applied where the value is used (outside, it reads as -1), so that the loads of row r + 2 issued during
-// row r are first waited for during row r + 1.
-template <typename Read>
-__device__ __forceinline__ void base_strip(const CambiPic &pic, Read read, unsigned bx)
+// Readers of the scale-0 samples.  col(x) and row(y) clamp a position into the picture and give its offsets (bytes, or
+// the coordinates for the anti-dithering reader); load(col, row) reads the sample there.  `step` advances row() by
+// one row.  The plain readers load through a buffer over the plane, so a row loop adds one scalar per row.
+struct BaseRead16 {  // 16-bit words holding the samples as they are (10-bit planar)
+    __amdgpu_buffer_rsrc_t rs;
+    int w, h;
+    unsigned step;
+    __device__ unsigned col(int x) const { return 2u * (unsigned)min(max(x, 0), w - 1); }
+    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
+    __device__ int load(unsigned c, unsigned r) const { return (int)__builtin_amdgcn_raw_buffer_load_b16(rs, c, r, 0); }
+};
+struct BaseRead16S : BaseRead16 {  // 16-bit words shifted: (word >> rshift) << shift (P010's high bits)
+    int rshift, shift;
+    __device__ int load(unsigned c, unsigned r) const
+    {
+        return (int)(((unsigned)__builtin_amdgcn_raw_buffer_load_b16(rs, c, r, 0) >> rshift) << shift);
+    }
+};
+struct BaseRead8 {  // bytes, shifted to the pipeline depth
+    __amdgpu_buffer_rsrc_t rs;
+    int w, h;
+    unsigned step;
+    int shift;
+    __device__ unsigned col(int x) const { return (unsigned)min(max(x, 0), w - 1); }
+    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
+    __device__ int load(unsigned c, unsigned r) const { return (int)((unsigned)__builtin_amdgcn_raw_buffer_load_b8(rs, c, r, 0) << shift); }
+};
+struct BaseReadFrom8 {  // cambi.c's 8-bit anti-dithering mean; offsets are the clamped coordinates
+    PlaneIn in;
+    int w, h;
+    unsigned step;
+    __device__ unsigned col(int x) const { return (unsigned)min(max(x, 0), w - 1); }
+    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1); }
+    __device__ int load(unsigned c, unsigned r) const { return (int)base_value(in, w, h, (int)c, (int)r); }
+};
+
+// One strip.  Positions outside the picture enter only as wave masks: margin columns outside it and rows outside it
+// have no flat bits, and the mode filter reads no sample outside the picture for any pixel it writes.  An interior
+// strip (kInterior: every row it reads lies inside the picture, rows 0, 1, h - 2 and h - 1 excepted) makes no row tests
+// and reads its rows at a running offset.  With kLoadOwn the lane's own column is loaded as well (cheap readers);
+// otherwise it is taken from the margin columns' samples by a lane swap.
+template <bool kLoadOwn, bool kInterior, typename Rd>
+__device__ __forceinline__ void base_strip(const CambiPic &pic, const Rd &rd, int x0, int y0, int y1)
 {
     const int w = pic.sw[0], h = pic.sh[0];
-    const int lane = threadIdx.x & 63, strip = (int)bx * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int xa = x0 - 32 + lane, xb = x0 + 32 + lane, x = x0 + lane;
+    const int index = pic.consts->mask_index;
+    // margin columns inside the picture, and the picture's last column (flat whatever its right neighbour)
+    const unsigned long long in_a = __builtin_amdgcn_ballot_w64(xa >= 0 && xa < w), in_b = __builtin_amdgcn_ballot_w64(xb < w);
+    const unsigned long long end_a = __builtin_amdgcn_ballot_w64(xa == w - 1), end_b = __builtin_amdgcn_ballot_w64(xb == w - 1);
+    const bool edge = x == 0 || x >= w - 1;  // the picture's first and last columns keep their own value
+    // columns x - 3 .. x + 3 are bits sft .. sft + 6 of the flat bits hi:lo of columns x0 - 32 .. x0 + 95
+    const int sft = lane + 29;
+    const bool low = sft < 64;
+    const unsigned va = rd.col(xa), vb = rd.col(xb), vl = rd.col(x);
+    // the packed plane as a buffer; a lane right of the picture stores past its end, where the store is dropped
+    const __amdgpu_buffer_rsrc_t out = buffer_over(pic.planes, (unsigned long long)w * h * 2);
+    const unsigned vo = x < w ? 2u * (unsigned)x : 0x80000000u;
+    // samples of rows r, r + 1 and r + 2 (loading) in slots (r - y0 + 3) mod 3 of A (columns a), B (columns b) and L
+    // (own column); the row loop is unrolled 3 times, so the slots rotate by renaming
+    int A[3], B[3], L[3] = {0, 0, 0};
+    {
+        const unsigned r0 = rd.row(y0 - 3), r1 = rd.row(y0 - 2);
+        A[0] = rd.load(va, r0);
+        B[0] = rd.load(vb, r0);
+        A[1] = rd.load(va, r1);
+        B[1] = rd.load(vb, r1);
+        if (kLoadOwn) {
+            L[0] = rd.load(vl, r0);
+            L[1] = rd.load(vl, r1);
+        }
+    }
+    unsigned ring = 0;  // the flat counts of the last 7 rows, 4 bits each, the newest lowest
+    int sum = 0;
+    int H[3] = {0, 0, 0};  // horizontal modes of rows r - 2 .. r in their rows' slots
+    int prev = 0;          // own value of row r - 1
+    unsigned md = 0;       // modes of rows r - 3 (low half) and r - 2 (high half), written 3 rows back
+    unsigned so = kInterior ? (unsigned)(y0 - 1) * rd.step : 0u;  // interior: offset of row r + 2
+    // row r in slot i; stores the packed value of row r - 3 when kStore
+    auto step = [&](const int r, const int i, const bool store) __attribute__((always_inline)) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;  // slots of rows r + 1 and r + 2 (= r - 1)
+        const int ca = A[i], cb = B[i], na = A[j], nb = B[j];
+        const unsigned rr = kInterior ? so : rd.row(r + 2);
+        A[k] = rd.load(va, rr);
+        B[k] = rd.load(vb, rr);
+        int cc = L[i];
+        if (kLoadOwn) {
+            L[k] = rd.load(vl, rr);
+        } else {
+            // the own column x0 + lane: column a of lane + 32 (lanes 0-31) or column b of lane - 32 (lanes 32-63)
+            const auto sw = __builtin_amdgcn_permlane32_swap((unsigned)ca, (unsigned)cb, false, false);
+            cc = (int)(lane < 32 ? sw[1] : sw[0]);
+        }
+        if (kInterior) so += rd.step;
+        // flat bits of row r: equal to the right neighbour (or the last column) and to the one below (or the last
+        // row); lane 63's right neighbour in a is lane 0 of b
+        const int ra = from_next_lane_or(ca, __builtin_amdgcn_readlane(cb, 0));
+        const int rb = from_next_lane(cb);
+        const unsigned long long ea = __builtin_amdgcn_ballot_w64(ca == ra), eb = __builtin_amdgcn_ballot_w64(cb == rb);
+        const unsigned long long da = __builtin_amdgcn_ballot_w64(ca == na), db = __builtin_amdgcn_ballot_w64(cb == nb);
+        unsigned long long lo, hi;
+        if (kInterior) {
+            lo = in_a & (end_a | ea) & da;
+            hi = in_b & (end_b | eb) & db;
+        } else {
+            lo = hi = 0;
+            if (r >= 0 && r < h) {
+                lo = in_a & (end_a | ea) & (r == h - 1 ? ~0ull : da);
+                hi = in_b & (end_b | eb) & (r == h - 1 ? ~0ull : db);
+            }
+        }
+        // the lane's 7 bits, without branching on the lane: both funnel forms, then a select
+        const unsigned w0 = (unsigned)(lo >> (sft & 63)) | (unsigned)(hi << ((64 - sft) & 63));
+        const unsigned w1 = (unsigned)(hi >> ((sft - 64) & 63));
+        const unsigned count = (unsigned)__builtin_popcount((low ? w0 : w1) & 0x7Fu);
+        sum += (int)count - (int)((ring >> 24) & 0xFu);
+        ring = ring << 4 | count;
+        // horizontal mode of row r at column x; every lane takes part in every shuffle, then the edge lanes take the
+        // margin columns' values
+        const int lft = from_prev_lane_or(cc, __builtin_amdgcn_readlane(ca, 31));
+        const int rgt = from_next_lane_or(cc, __builtin_amdgcn_readlane(cb, 32));
+        int hv = (int)mode3(lft, cc, rgt);
+        asm("" : "+v"(hv));  // computed on every lane and selected, rather than branched around on the edge lanes
+        const int h3 = edge ? cc : hv;
+        // the mode of row r - 1 from rows r - 2 (slot j), r - 1 (slot k) and r; its first and last rows are copied
+        const int ym = r - 1;
+        const int m = !kInterior && (ym == 0 || ym == h - 1) ? prev : (int)mode3(H[j], H[k], h3);
+        H[i] = h3;
+        prev = cc;
+        if (store) {  // the row whose 7x7 count is complete, r - 3; its mode is the low half of md
+            const unsigned y = (unsigned)(r - 3);
+            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(md | ((sum > index ? 1u : 0u) << 15)), out, vo,
+                                                  __builtin_amdgcn_readfirstlane(2u * y * (unsigned)w), 0);
+        }
+        md = __builtin_amdgcn_alignbit((unsigned)m, md, 16);  // (m : md) >> 16: row r - 2's mode low, row r - 1's high
+    };
+    // rows y0 - 3 .. y0 + 2 fill the window; then rows y0 + 3 .. y1 + 2 write rows y0 .. y1 - 1, three at a time while
+    // three remain; the slot of row r is (r - y0 + 3) mod 3 throughout
+    step(y0 - 3, 0, false);
+    step(y0 - 2, 1, false);
+    step(y0 - 1, 2, false);
+    step(y0, 0, false);
+    step(y0 + 1, 1, false);
+    step(y0 + 2, 2, false);
+    int r = y0 + 3;
+    for (; r + 2 < y1 + 3; r += 3) {
+        step(r, 0, true);
+        step(r + 1, 1, true);
+        step(r + 2, 2, true);
+    }
+    if (r < y1 + 3) step(r, 0, true);
+    if (r + 1 < y1 + 3) step(r + 1, 1, true);
+}
+
+// A strip of the reader's plane: interior strips take the row-test-free form.
+template <bool kLoadOwn, typename Rd>
+__device__ __forceinline__ void base_strip_any(const CambiPic &pic, const Rd &rd, unsigned bx)
+{
+    const int w = pic.sw[0], h = pic.sh[0];
+    const int strip = (int)bx * 4 + wave_index();
     const int across = (w + kBaseW - 1) / kBaseW;
     if (strip >= across * ((h + kBaseRows - 1) / kBaseRows)) return;  // uniform over the wave
     const int x0 = (strip % across) * kBaseW, y0 = (strip / across) * kBaseRows;
     const int y1 = min(y0 + kBaseRows, h);
-    const int xa = x0 - 32 + lane, xb = x0 + 32 + lane, x = x0 + lane;
-    const int index = pic.consts->mask_index;
-    // rows r - 3 .. r + 3 of the mask window end at r + 3; the flat bits of row r need row r + 1, which was read
-    // (issued) one row earlier
-    const bool ina = xa >= 0 && xa < w, inb = xb >= 0 && xb < w, inc = x < w;
-    auto at = [&](int v, bool in, int yy) { return in && yy >= 0 && yy < h ? v : -1; };
-    int ca = at(read(xa, y0 - 3), ina, y0 - 3), cb = at(read(xb, y0 - 3), inb, y0 - 3), cc = at(read(x, y0 - 3), inc, y0 - 3);
-    int qa = read(xa, y0 - 2), qb = read(xb, y0 - 2), qc = read(x, y0 - 2);
-    int last[7] = {0, 0, 0, 0, 0, 0, 0}, sum = 0;
-    int hm[3] = {0, 0, 0}, cv[3] = {0, 0, 0};  // horizontal modes and own values of rows r - 2 .. r
-    int md[3] = {0, 0, 0};                      // modes of rows r - 3 .. r - 1 (written 3 rows back)
-    for (int r = y0 - 3; r < y1 + 3; r++) {
-        const int na = at(qa, ina, r + 1), nb = at(qb, inb, r + 1), nc = at(qc, inc, r + 1);
-        qa = read(xa, r + 2);
-        qb = read(xb, r + 2);
-        qc = read(x, r + 2);
-        // flat bits of row r over columns x0 - 32 .. x0 + 95
-        const int ra0 = from_next_lane(ca), rb = from_next_lane(cb), b0 = __builtin_amdgcn_readlane(cb, 0);
-        const int ra = lane == 63 ? b0 : ra0;
-        const bool fa = ca >= 0 && (xa == w - 1 || ca == ra) && (r == h - 1 || ca == na);
-        const bool fb = cb >= 0 && (xb == w - 1 || cb == rb) && (r == h - 1 || cb == nb);
-        const unsigned long long lo = __ballot(fa), hi = __ballot(fb);
-        // columns x - 3 .. x + 3 are bits lane + 29 .. lane + 35 of hi:lo
-        const int sft = lane + 29;
-        const unsigned long long win = sft < 64 ? (lo >> sft) | (hi << (64 - sft)) : hi >> (sft - 64);
-        const int count = __popcll(win & 0x7Full);
-        sum += count - last[0];
-#pragma unroll
-        for (int k = 0; k < 6; k++) last[k] = last[k + 1];
-        last[6] = count;
-        // horizontal mode of row r at column x; the first and last columns keep their own value
-        // every lane takes part in every shuffle; the edge lanes then pick the margin columns' values
-        const int left = from_prev_lane(cc), right = from_next_lane(cc);
-        const int a31 = __builtin_amdgcn_readlane(ca, 31), b32 = __builtin_amdgcn_readlane(cb, 32);
-        const int lft = lane == 0 ? a31 : left, rgt = lane == 63 ? b32 : right;
-        const int h3 = (x == 0 || x >= w - 1) ? cc : (int)mode3((uint16_t)lft, (uint16_t)cc, (uint16_t)rgt);
-        hm[0] = hm[1];
-        hm[1] = hm[2];
-        hm[2] = h3;
-        cv[0] = cv[1];
-        cv[1] = cv[2];
-        cv[2] = cc;
-        // the mode of row r - 1: its first and last rows are copied
-        const int ym = r - 1;
-        const int m = (ym == 0 || ym == h - 1) ? cv[1] : (int)mode3((uint16_t)hm[0], (uint16_t)hm[1], (uint16_t)hm[2]);
-        md[0] = md[1];
-        md[1] = md[2];
-        md[2] = m;
-        const int y = r - 3;  // the row whose 7x7 count is complete; its mode is md[0]
-        if (y >= y0 && y < y1 && x < w)
-            ((ES_GLOBAL uint16_t *)pic.planes)[(size_t)y * w + x] = (uint16_t)(md[0] | ((sum > index ? 1 : 0) << 15));
-        ca = na;
-        cb = nb;
-        cc = nc;
-    }
+    // rows read: y0 - 3 .. y1 + 4, all inside the picture; rows tested for flatness: y0 - 3 .. y1 + 2, none the last;
+    // rows filtered: y0 .. y1 - 1, neither the first nor the last
+    if (y0 - 3 >= 2 && y1 + 4 <= h - 2) base_strip<kLoadOwn, true>(pic, rd, x0, y0, y1);
+    else base_strip<kLoadOwn, false>(pic, rd, x0, y0, y1);
 }
 
 __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
@@ -129,17 +234,15 @@ __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
     const CambiPic &pic = pics[b.y];
     const int w = pic.sw[0], h = pic.sh[0];
     const PlaneIn in = pic.in;
-    auto cx = [=](int xx) { return min(max(xx, 0), w - 1); };
-    auto cy = [=](int yy) { return (size_t)min(max(yy, 0), h - 1) * in.pitch; };
     // one reader per plane format, chosen once per block
     if (in.from8) {
-        base_strip(pic, [=](int xx, int yy) { return (int)base_value(in, w, h, cx(xx), min(max(yy, 0), h - 1)); }, b.x);
+        base_strip_any<false>(pic, BaseReadFrom8{in, w, h, 1u}, b.x);
     } else if (in.bytes == 1) {
-        const ES_GLOBAL uint8_t *p = (const ES_GLOBAL uint8_t *)in.ptr;
-        base_strip(pic, [=](int xx, int yy) { return (int)((unsigned)p[cy(yy) + cx(xx)] << in.shift); }, b.x);
+        base_strip_any<true>(pic, BaseRead8{buffer_over(in.ptr, (unsigned long long)in.pitch * h), w, h, (unsigned)in.pitch, in.shift}, b.x);
     } else {
-        const ES_GLOBAL uint16_t *p = (const ES_GLOBAL uint16_t *)in.ptr;
-        base_strip(pic, [=](int xx, int yy) { return (int)(((unsigned)p[cy(yy) + cx(xx)] >> in.rshift) << in.shift); }, b.x);
+        const BaseRead16 r16{buffer_over(in.ptr, (unsigned long long)in.pitch * h * 2), w, h, 2u * (unsigned)in.pitch};
+        if (in.rshift == 0 && in.shift == 0) base_strip_any<true>(pic, r16, b.x);  // 10-bit samples, the common case
+        else base_strip_any<true>(pic, BaseRead16S{r16, in.rshift, in.shift}, b.x);
     }
 }
 
@@ -165,7 +268,7 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
     const uint3 b = xcd_block3();  // neighbouring strips share halo rows
     const CambiPic &pic = pics[b.y];
     const int w = pic.sw[s], h = pic.sh[s], pw = pic.sw[s - 1];
-    const int lane = threadIdx.x & 63, strip = (int)b.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, strip = (int)b.x * 4 + wave_index();
     const int across = (w + 63) / 64;
     if (strip >= across * ((h + kScaleRows - 1) / kScaleRows)) return;  // uniform over the wave
     const int x0 = (strip % across) * 64, y0 = (strip / across) * kScaleRows, y1 = min(y0 + kScaleRows, h);
@@ -180,7 +283,7 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
         const unsigned v = c & 0x7FFFu, e = hv & 0x7FFFu;
         const unsigned left = (unsigned)from_prev_lane((int)v), right = (unsigned)from_next_lane((int)v);
         const unsigned l = lane == 0 ? e : left, rt = lane == 63 ? e : right;
-        return (x == 0 || x >= w - 1) ? v : (unsigned)mode3((uint16_t)l, (uint16_t)v, (uint16_t)rt);
+        return (x == 0 || x >= w - 1) ? v : mode3(l, v, rt);
     };
     // rows y0 - 1 and y0 to start: t = horizontal modes, c = raw samples (value and mask bit)
     unsigned c0 = prev[row(y0 - 1) + xc], e0 = prev[row(y0 - 1) + xh];
@@ -193,7 +296,7 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
         nc = prev[row(y + 2) + xc];
         ne = prev[row(y + 2) + xh];
         const unsigned t2 = hmode(c2, e2);
-        const unsigned m = (y == 0 || y == h - 1) ? (c1 & 0x7FFFu) : (unsigned)mode3((uint16_t)t0, (uint16_t)t1, (uint16_t)t2);
+        const unsigned m = (y == 0 || y == h - 1) ? (c1 & 0x7FFFu) : mode3(t0, t1, t2);
         if (x < w) out[(size_t)y * w + x] = (uint16_t)(m | (c1 & 0x8000u));
         t0 = t1;
         t1 = t2;
@@ -688,14 +791,7 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     CvTile t;
     t.w = pic.sw[s];
     t.h = pic.sh[s];
-    {
-        // one tile per wave: the plane's address is the same on every lane, which keeps the descriptor in scalar
-        // registers
-        const unsigned long long a = (unsigned long long)(pic.planes + pic.off[s]);
-        const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-        t.rs = __builtin_amdgcn_make_buffer_rsrc((void *)((unsigned long long)hi << 32 | lo), (short)0,
-                                                 __builtin_amdgcn_readfirstlane(t.w * t.h * 2), 0x00020000);
-    }
+    t.rs = buffer_over(pic.planes + pic.off[s], (unsigned long long)t.w * t.h * 2);  // one tile per wave
     const int across = (t.w + kCvTileW - 1) / kCvTileW;
     t.x0 = (local % across) * kCvTileW;
     t.y0 = (local / across) * tile_h;

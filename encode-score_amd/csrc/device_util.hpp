@@ -21,6 +21,22 @@ __device__ __forceinline__ uint3 xcd_block3()
     return make_uint3(l % gx, (l / gx) % gy, l / (gx * gy));
 }
 
+// The wave's index in its block as a wave-uniform value.  The compiler cannot tell that threadIdx.x >> 6 is the same
+// on every lane of a 64-wide wave, and would otherwise keep what is derived from it (a strip's origin, its row
+// counter, row addresses and row tests) in vector registers and compute it per lane.
+__device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
+// `bytes` bytes at p as a raw buffer (the address made wave-uniform).  Loads through it take a byte offset per lane
+// and one in a scalar register, so a loop over rows adds one scalar per row instead of a 64-bit address per lane; a
+// load past `bytes` returns 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(const void *p, unsigned long long bytes)
+{
+    const unsigned long long a = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+    const unsigned n = __builtin_amdgcn_readfirstlane((unsigned)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull));
+    return __builtin_amdgcn_make_buffer_rsrc((void *)((unsigned long long)hi << 32 | lo), (short)0, (int)n, 0x00020000);
+}
+
 __device__ __forceinline__ unsigned raw_sample(const PlaneIn &in, int x, int y)
 {
     const size_t i = (size_t)y * in.pitch + x;

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void window_stats_kernel(const WsPair *pairs)
 {
     __shared__ int rowbuf[2][4][2][64 + 8];  // double-buffered rows, per wave and picture
     const WsPair &P = pairs[blockIdx.z];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_index();
     const int w = P.w, h = P.h, ww = w / WIN, wh = h / WIN, W = ww * WIN;
     const int x0 = (blockIdx.x * 4 + wave) * 64, wy0 = blockIdx.y * kWsRows;
     if (wy0 >= wh || blockIdx.x * 256 >= W) return;  // uniform over the block
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
 {
     const uint3 b = xcd_block3();  // neighbouring patches share cache lines
     const LastingJob &J = jobs[b.z];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wave = wave_index();
     const int ww = J.w / WIN, wh = J.h / WIN, pw = ww * (WIN / 16), ph = wh * (WIN / 16);
     const int patch = (int)b.x * 4 + wave;
     if (J.ncentres <= 0 || patch >= pw * ph) return;  // uniform over the wave
