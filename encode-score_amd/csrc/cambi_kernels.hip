@@ -354,6 +354,7 @@ struct CvTile {
     int tvi[kDiffs], weight[kDiffs];
     ES_GLOBAL float *plane;  // full c-value plane or nullptr
     ES_GLOBAL float *seg;    // this tile's segment of the list
+    __amdgpu_buffer_rsrc_t plane_rs, seg_rs;  // the same as buffers (no plane: an empty one)
 };
 
 // Samples are read as buffer loads at byte offset xo + yo: a load past the buffer's w * h samples returns 0, which is
@@ -587,9 +588,10 @@ constexpr int kMWin = 68;   // u16 per number in the window table: 64 columns an
 
 template <int NB>
 struct CvSharedM {
-    uint8_t cnt[32 * NB][kMCnt];  // column counts of number n at margin column c
-    uint16_t win[32 * NB][kMWin];  // window counts of number n at output column x
+    uint8_t cnt[32 * NB + 1][kMCnt];   // column counts of number n at margin column c; row B: the pass's discards
+    uint16_t win[32 * NB + 1][kMWin];  // window counts of number n at output column x; row B: zeros
     uint16_t rank[kMaxBand + 2 * kDiffs];
+    uint8_t row[kMaxBand + 2 * kDiffs];  // the pass's row of each value: its number less b0, or B outside the pass
     __device__ uint8_t *flags() { return (uint8_t *)&win[0][0]; }
 };
 static_assert(sizeof(uint16_t) * 32 * kMWin >= 2 * kCvFlags, "flags live in the window table during the setup");
@@ -605,11 +607,14 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
     return v;
 }
 
-// Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds).
-template <int NB, int NBU>
+// Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds); the band of every output block
+// touches NJ margin blocks.  Per pass each value's row in the tables is looked up once (row[]), its number less b0
+// or B when it is outside the pass or not relevant: count updates of such samples go to the discard row B of cnt,
+// and window counts read from the zero row B of win, so the row loop makes no range tests.
+template <int NB, int NBU, int NJ>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
 {
-    static_assert(NBU >= 1 && NBU <= NB, "blocks of numbers");
+    static_assert(NBU >= 1 && NBU <= NB && NJ >= 1 && NJ <= 3, "blocks of numbers, band blocks");
     constexpr int B = 32 * NBU;
     const int lane = threadIdx.x, pad = t.pad, span = 2 * pad;
     const int c0 = 2 * lane, xl = t.x0 - pad + c0;
@@ -617,26 +622,26 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
     const int x = t.x0 + lane;
     const unsigned xo0 = col0 ? cv_col(t, xl) : kCvOut, xo1 = col1 ? cv_col(t, xl + 1) : kCvOut, xoo = cv_col(t, x);
     // band operands of output block mb and margin block mb + j: this lane's row 32 mb + lane % 32 and its half of
-    // the block's columns; nj blocks per output block carry the band
+    // the block's columns
     const int g = lane >> 5, r32 = lane & 31;
-    const int nj = __builtin_amdgcn_readfirstlane(min((31 + span) / 32 + 1, 3));  // the same on every lane
-    cv_v4i band[2][3];
+    cv_v4i band[2][NJ];
 #pragma unroll
     for (int mb = 0; mb < 2; mb++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) band[mb][j] = band_bytes(32 * mb + r32, 32 * (mb + j) + 16 * g, span);
-    uint32_t *cnt_words = (uint32_t *)&sh.cnt[0][0];
+        for (int j = 0; j < NJ; j++) band[mb][j] = band_bytes(32 * mb + r32, 32 * (mb + j) + 16 * g, span);
+    // a lane's two margin columns share a count word of every number's row: its byte offset in the row, and the
+    // increments of its columns' bytes (a decrement is the negated increment)
+    const unsigned cword = (unsigned)(c0 >> 2) * 4u;
+    const unsigned inc0 = 1u << ((c0 & 3) * 8), inc1 = 1u << (((c0 + 1) & 3) * 8);
+    uint8_t *const cnt = &sh.cnt[0][0];
+    auto bump = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), inc); };
+    const int nv = t.band_size + 2 * kDiffs;
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
-        for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)&sh.cnt[0][0])[i] = cv_v4i{0, 0, 0, 0};
-        // A sample of rank rk entering (delta 1) or leaving (delta ~0) column c's count: an increment of zero when
-        // it is inactive or outside the pass, so that every lane makes the same LDS operation.  A lane's two columns
-        // share a word of every number's row.
-        auto bump = [&](unsigned rk, int c, unsigned delta) {
-            const unsigned b = rk - (unsigned)b0;
-            const unsigned inc = b < (unsigned)B ? delta << ((c & 3) << 3) : 0u;
-            atomicAdd(cnt_words + (b & (B - 1)) * (kMCnt / 4) + (c >> 2), inc);
-        };
+        for (int i = lane; i < (B + 1) * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
+        if (lane < kMWin / 2) ((uint32_t *)&sh.win[B][0])[lane] = 0u;
+        for (int i = lane; i < nv; i += 64) sh.row[i] = (uint8_t)min((unsigned)sh.rank[i] - (unsigned)b0, (unsigned)B);
+        __syncthreads();  // one wave: orders the table writes before the reads below
         const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
         for (int y = ya; y < yb; y += 8) {
             unsigned v[16];
@@ -646,13 +651,13 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                 v[2 * j] = cv_at(t, xo0, yy);
                 v[2 * j + 1] = cv_at(t, xo1, yy);
             }
-            unsigned rk[16];
+            unsigned rw[16];
 #pragma unroll
-            for (int j = 0; j < 16; j++) rk[j] = sh.rank[cv_index(t, v[j])];
+            for (int j = 0; j < 16; j++) rw[j] = sh.row[cv_index(t, v[j])];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                bump(rk[2 * j], c0, 1u);
-                bump(rk[2 * j + 1], c0 + 1, 1u);
+                bump(rw[2 * j], inc0);
+                bump(rw[2 * j + 1], inc1);
             }
         }
         auto load = [&](int y, unsigned *v) {
@@ -664,40 +669,41 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
         };
         unsigned cur[5], nxt[5] = {0, 0, 0, 0, 0}, gone0 = 0, gone1 = 0;
         load(t.y0, cur);
+        const unsigned lo_rel = (unsigned)(lo - b0), hi_rel = (unsigned)(hi - b0);
         for (int r = 0; r < t.rows; r++) {
             const int y = t.y0 + r;
             if (r + 1 < t.rows) load(y + 1, nxt);
             const unsigned io = cv_index(t, cur[2]), base = io ? io - kDiffs : 0u;
-            const unsigned re0 = sh.rank[cv_index(t, cur[0])], re1 = sh.rank[cv_index(t, cur[1])];
-            const unsigned rl0 = sh.rank[cv_index(t, gone0)], rl1 = sh.rank[cv_index(t, gone1)];
+            const unsigned re0 = sh.row[cv_index(t, cur[0])], re1 = sh.row[cv_index(t, cur[1])];
+            const unsigned rl0 = sh.row[cv_index(t, gone0)], rl1 = sh.row[cv_index(t, gone1)];
             unsigned rk[2 * kDiffs + 1];
 #pragma unroll
-            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = sh.rank[base + k];
-            bump(rl0, c0, ~0u);
-            bump(rl1, c0 + 1, ~0u);
-            bump(re0, c0, 1u);
-            bump(re1, c0 + 1, 1u);
-            const unsigned own = io ? rk[kDiffs] : (unsigned)kNone;
-            const bool serve = own != kNone && (int)own >= lo && (int)own < hi;
+            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = sh.row[base + k];
+            bump(rl0, 0u - inc0);
+            bump(rl1, 0u - inc1);
+            bump(re0, inc0);
+            bump(re1, inc1);
+            const unsigned own = io ? rk[kDiffs] : (unsigned)B;
+            const bool serve = own >= lo_rel && own < hi_rel;
             if (__builtin_amdgcn_ballot_w64(serve)) {
                 // the LDS operations of one wave complete in order: the row's updates are in before these reads
                 asm volatile("" ::: "memory");
-                cv_v4i bop[NBU][4];
+                cv_v4i bop[NBU][NJ + 1];
 #pragma unroll
                 for (int nb = 0; nb < NBU; nb++)
 #pragma unroll
-                    for (int kb = 0; kb < 4; kb++)
-                        bop[nb][kb] = kb <= nj ? *(const cv_v4i *)&sh.cnt[32 * nb + r32][32 * kb + 16 * g] : cv_v4i{0, 0, 0, 0};
+                    for (int kb = 0; kb <= NJ; kb++) bop[nb][kb] = *(const cv_v4i *)&sh.cnt[32 * nb + r32][32 * kb + 16 * g];
 #pragma unroll
                 for (int nb = 0; nb < NBU; nb++) {
                     // the two output blocks' products interleaved, so that neither waits on its own previous one
-                    cv_v16i acc[2] = {};
+                    cv_v16i acc[2];
 #pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        if (j < nj)
+                    for (int mb = 0; mb < 2; mb++) acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][0], bop[nb][mb], cv_v16i{}, 0, 0, 0);
 #pragma unroll
-                            for (int mb = 0; mb < 2; mb++)
-                                acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc[mb], 0, 0, 0);
+                    for (int j = 1; j < NJ; j++)
+#pragma unroll
+                        for (int mb = 0; mb < 2; mb++)
+                            acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc[mb], 0, 0, 0);
                     // D's register 4 q + i of this lane: output column 32 mb + 8 q + 4 (lane / 32) + i, number
                     // 32 nb + lane % 32; four columns per 8-byte store
 #pragma unroll
@@ -710,34 +716,34 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                         }
                 }
                 asm volatile("" ::: "memory");
-                unsigned cnt[2 * kDiffs + 1];
+                unsigned cnt9[2 * kDiffs + 1];
 #pragma unroll
-                for (int k = 0; k < 2 * kDiffs + 1; k++) {
-                    const unsigned b = rk[k] - (unsigned)b0, bb = b & (B - 1);
-                    const unsigned d = sh.win[bb][lane];
-                    cnt[k] = b < (unsigned)B ? d : 0u;
-                }
-#pragma unroll
-                for (int k = 0; k < 2 * kDiffs + 1; k++) asm volatile("" ::"v"(cnt[k]));
+                for (int k = 0; k < 2 * kDiffs + 1; k++) cnt9[k] = sh.win[rk[k]][lane];
                 const int value = (int)(io ? io - kDiffs : 0u) + t.band_base + kDiffs;
-                const unsigned p0 = cnt[kDiffs];
+                const unsigned p0 = cnt9[kDiffs];
                 float c = 0.0f;
 #pragma unroll
                 for (int d = 0; d < kDiffs; d++) {
-                    const unsigned pa = cnt[kDiffs + d + 1], pb = cnt[kDiffs - d - 1];
+                    const unsigned pa = cnt9[kDiffs + d + 1], pb = cnt9[kDiffs - d - 1];
                     const unsigned pm = pa > pb ? pa : pb;
+                    // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; computed on
+                    // every lane and selected (p0 + pm is at least 1 on a served pixel, its own count)
                     const float rcp = recip_rn(p0 + pm);
-                    const float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
+                    float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
+                    asm("" : "+v"(val));
                     const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt && pm > 0;
                     c = on && val > c ? val : c;
                 }
                 c = serve ? c : 0.0f;
-                if (t.plane) {
-                    if (serve) t.plane[(unsigned)(y * t.w + x)] = c;
-                }
+                // the full plane (when kept) and the tile's list, as buffer stores: a lane with nothing to store
+                // stores past the end, where the store is dropped
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.plane_rs, serve ? 4u * (unsigned)x : 0x80000000u,
+                                                      __builtin_amdgcn_readfirstlane(4u * (unsigned)y * (unsigned)t.w), 0);
                 sum_fx += c > 0.0f ? fixed24(c) : 0ull;
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
-                if (c > 0.0f) t.seg[tcount + __popcll(m & ((1ull << lane) - 1))] = c;
+                const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.seg_rs, c > 0.0f ? 4u * below : 0x80000000u,
+                                                      __builtin_amdgcn_readfirstlane(4u * tcount), 0);
                 tcount += (unsigned)__popcll(m);
                 asm volatile("" ::: "memory");  // every read of the table before the next row writes it
             }
@@ -748,6 +754,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
         }
         if (hi == relevant) break;
         b0 = hi - kDiffs;
+        __syncthreads();  // the pass's reads of the tables are done before the next pass rewrites them
     }
 }
 
@@ -765,12 +772,22 @@ __device__ __forceinline__ void cv_run(CvShared<BMAX> &sh, const CvTile &t, int 
     else if (BMAX == 32 || relevant <= 32) cv_passes<32>(sh, t, relevant, sum_fx, tcount);
     else cv_passes<(BMAX > 32 ? BMAX : 32)>(sh, t, relevant, sum_fx, tcount);
 }
+template <int NB, int NJ>
+__device__ __forceinline__ void cv_run_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
+                                         unsigned &tcount)
+{
+    if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ>(sh, t, relevant, sum_fx, tcount);
+    else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ>(sh, t, relevant, sum_fx, tcount);
+}
 template <int NB>
 __device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
                                        unsigned &tcount)
 {
-    if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1>(sh, t, relevant, sum_fx, tcount);
-    else cv_passes_m<NB, (NB > 1 ? 2 : 1)>(sh, t, relevant, sum_fx, tcount);
+    // the margin blocks the band of an output block touches: 2 pad + 1 columns from the block's first
+    const int nj = min((31 + 2 * t.pad) / 32 + 1, 3);
+    if (nj == 3) cv_run_m<NB, 3>(sh, t, relevant, sum_fx, tcount);
+    else if (nj == 2) cv_run_m<NB, 2>(sh, t, relevant, sum_fx, tcount);
+    else cv_run_m<NB, 1>(sh, t, relevant, sum_fx, tcount);
 }
 
 __device__ int find_slot(const int *first, int n, int t)
@@ -796,17 +813,20 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     t.x0 = (local % across) * kCvTileW;
     t.y0 = (local / across) * tile_h;
     t.rows = min(tile_h, t.h - t.y0);
-    t.pad = K.pad;
-    t.band_base = K.band_base;
-    t.band_size = K.band_size;
-    t.vlt = K.vlt;
+    // the picture's constants, the same on every lane
+    t.pad = __builtin_amdgcn_readfirstlane(K.pad);
+    t.band_base = __builtin_amdgcn_readfirstlane(K.band_base);
+    t.band_size = __builtin_amdgcn_readfirstlane(K.band_size);
+    t.vlt = __builtin_amdgcn_readfirstlane(K.vlt);
 #pragma unroll
     for (int d = 0; d < kDiffs; d++) {
-        t.tvi[d] = K.tvi[d];
-        t.weight[d] = K.weight[d];
+        t.tvi[d] = __builtin_amdgcn_readfirstlane(K.tvi[d]);
+        t.weight[d] = __builtin_amdgcn_readfirstlane(K.weight[d]);
     }
     t.plane = pic.cvals ? (ES_GLOBAL float *)(pic.cvals + pic.off[s]) : nullptr;
     t.seg = (ES_GLOBAL float *)(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h);
+    t.plane_rs = buffer_over(pic.cvals ? pic.cvals + pic.off[s] : nullptr, pic.cvals ? (unsigned long long)t.w * t.h * 4 : 0);
+    t.seg_rs = buffer_over(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h, (unsigned long long)kCvTileW * tile_h * 4);
     unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
     const int lane = threadIdx.x, nv = t.band_size + 2 * kDiffs;
     const int c0 = 2 * lane, xl = t.x0 - t.pad + c0, cw = kCvTileW + 2 * t.pad;

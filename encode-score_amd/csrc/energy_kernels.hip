@@ -168,21 +168,35 @@ __global__ __launch_bounds__(256) void window_stats_kernel(const WsPair *pairs)
 constexpr int kWsStrip = 64;
 
 
-// A sample of a plane of BYTES bytes per sample (1 or 2), shifted to the pipeline depth.
-template <int BYTES>
-__device__ __forceinline__ int plane_sample(const PlaneIn &in, int x, int y)
-{
-    const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = BYTES == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)in.ptr)[i]
-                                  : (unsigned)((const ES_GLOBAL uint16_t *)in.ptr)[i] >> in.rshift;
-    return (int)(v << in.shift);
-}
+// Samples of a plane of BYTES bytes per sample (1 or 2) shifted to the pipeline depth, (word >> rshift) << lsh with
+// lsh the plane's shift plus the pair's; kShift false: no shifts (16-bit words holding the samples as they are).  The
+// plane is read as a buffer: the lane's column offset is fixed and the row's offset is one scalar.
+template <int BYTES, bool kShift>
+struct WsRead {
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned step;  // bytes per row
+    int h, rshift, lsh;
+    __device__ WsRead(const PlaneIn &in, int h_, int up)
+        : rs(buffer_over(in.ptr, (unsigned long long)in.pitch * h_ * BYTES)), step((unsigned)in.pitch * BYTES), h(h_),
+          rshift(in.rshift), lsh(in.shift + up) {}
+    __device__ unsigned col(int x) const { return (unsigned)x * BYTES; }
+    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
+    __device__ int load(unsigned c, unsigned r) const
+    {
+        unsigned v = BYTES == 1 ? (unsigned)__builtin_amdgcn_raw_buffer_load_b8(rs, c, r, 0)
+                                : (unsigned)__builtin_amdgcn_raw_buffer_load_b16(rs, c, r, 0);
+        if (kShift) v = (v >> rshift) << lsh;
+        return (int)v;
+    }
+};
 
 typedef int WsRows[2][2][72 + 64];  // [buffer][picture][column - (x0 - 4)]; 72 .. 135 take unused halo stores
+typedef long long WsRed[6][64];     // a group's per-lane sums on their way to the window sums
 
-// One strip of the pair; BR and BD are the source's and the encode's bytes per sample.
-template <int WIN, int BR, int BD>
-__device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsigned bx, unsigned by)
+// One strip of the pair; BR and BD are the source's and the encode's bytes per sample, kShift whether either plane's
+// samples are shifted.
+template <int WIN, int BR, int BD, bool kShift>
+__device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed &red, unsigned bx, unsigned by)
 {
     static_assert(kWsStrip % WIN == 0 && WIN % 16 == 0, "windows tile the strip in 16-row blocks");
     const int lane = threadIdx.x;
@@ -196,17 +210,20 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsign
     const int xa = min(x, w - 1);
     const int xh = lane < 4 ? max(x0 - 4 + lane, 0) : min(x0 + 60 + lane, w - 1);
     const int hslot = lane < 4 ? lane : lane < 8 ? lane + 64 : 72 + lane;
-    const PlaneIn in[2] = {P.r, P.d};
-    int32_t *mid[2] = {P.mid_r, P.mid_d};
-    auto sample = [&](int p, int xx, int r) {
-        const int yy = min(max(r, 0), h - 1);
-        return (p == 0 ? plane_sample<BR>(in[0], xx, yy) : plane_sample<BD>(in[1], xx, yy)) << P.up;
-    };
+    const WsRead<BR, kShift> rd0(P.r, h, P.up);
+    const WsRead<BD, kShift> rd1(P.d, h, P.up);
+    const unsigned ca[2] = {rd0.col(xa), rd1.col(xa)}, ch[2] = {rd0.col(xh), rd1.col(xh)};
+    auto sample = [&](int p, unsigned c, int r) { return p == 0 ? rd0.load(c, rd0.row(r)) : rd1.load(c, rd1.row(r)); };
+    // the M planes as buffers (none: an empty one, whose stores are dropped); a lane right of the covered width stores
+    // past the end
+    const __amdgpu_buffer_rsrc_t mid[2] = {buffer_over(P.mid_r, P.mid_r ? (unsigned long long)w * h * 4 : 0),
+                                           buffer_over(P.mid_d, P.mid_d ? (unsigned long long)w * h * 4 : 0)};
+    const unsigned mo = inside ? 4u * (unsigned)x : 0x80000000u;
     int pr[2], ph[2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-        pr[p] = sample(p, xa, y0 - 4);
-        ph[p] = sample(p, xh, y0 - 4);
+        pr[p] = sample(p, ca[p], y0 - 4);
+        ph[p] = sample(p, ch[p], y0 - 4);
     }
     int h3c[2][9], h3l[2][3], h3r[2][3], h9[2][9], yv[2][3], s9[2];
 #pragma unroll
@@ -220,8 +237,10 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsign
     int accF[3] = {0, 0, 0}, accG[6] = {0, 0, 0, 0, 0, 0};
     long long accM[3] = {0, 0, 0};
     long long winF[3] = {0, 0, 0}, winG[6] = {0, 0, 0, 0, 0, 0}, winM[3] = {0, 0, 0};  // WIN > 16 only
-    // a closed 16-row block of one group: straight to its windows (WIN = 16) or into the lane's window sums
-    auto flush = [&](auto &acc, auto &win, const auto &slot, int y) {
+    // a closed 16-row block of one group: into the lane's window sums, and when the block closes a row of windows, the
+    // window sums: each lane leaves its N sums in LDS, then lane k of each window's WIN lanes adds up sum k of them
+    // and writes it to slot sbase + k sstride of the window
+    auto flush = [&](auto &acc, auto &win, const int sbase, const int sstride, int y) {
         constexpr int N = sizeof(acc) / sizeof(acc[0]);
 #pragma unroll
         for (int k = 0; k < N; k++) {
@@ -229,17 +248,25 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsign
             acc[k] = 0;
         }
         if ((y + 1) % WIN) return;  // uniform
+        __syncthreads();  // one wave per block: orders the LDS traffic only
 #pragma unroll
-        for (int k = 0; k < N; k++) win[k] = reduce_lanes(win[k], WIN);
-        if (inside && lane % WIN == 0) {
-            ES_GLOBAL long long *o = (ES_GLOBAL long long *)P.sums + ((size_t)(y / WIN) * ww + x / WIN) * 12;
-#pragma unroll
-            for (int k = 0; k < N; k++) o[slot[k]] = win[k];
+        for (int k = 0; k < N; k++) {
+            red[k][lane] = win[k];
+            win[k] = 0;
         }
+        __syncthreads();
+        const int g0 = lane & ~(WIN - 1), k = lane & (WIN - 1);
+        if (k < N) {
+            long long t = 0;
 #pragma unroll
-        for (int k = 0; k < N; k++) win[k] = 0;
+            for (int j = 0; j < WIN; j += 2) {
+                const longlong2 v = *(const longlong2 *)&red[k][g0 + j];
+                t += v.x + v.y;
+            }
+            if (x0 + g0 < W)
+                ((ES_GLOBAL long long *)P.sums)[((size_t)(y / WIN) * ww + (x0 + g0) / WIN) * 12 + sbase + k * sstride] = t;
+        }
     };
-    const int slotF[3] = {0, 2, 4}, slotG[6] = {6, 7, 8, 9, 10, 11}, slotM[3] = {1, 3, 5};
     int buf = 0;
     for (int base = y0 - 4; base <= rend; base += 9) {
 #pragma unroll
@@ -253,8 +280,8 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsign
             }
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                pr[p] = sample(p, xa, r + 1);
-                ph[p] = sample(p, xh, r + 1);
+                pr[p] = sample(p, ca[p], r + 1);
+                ph[p] = sample(p, ch[p], r + 1);
             }
             __syncthreads();  // the row is in (one wave per block: no other wave waits here)
             int F[2], GX[2], GY[2], M[2];
@@ -290,7 +317,7 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsign
                 accF[0] = __mul24(F[0], F[0]) + accF[0];
                 accF[1] = __mul24(F[1], F[1]) + accF[1];
                 accF[2] = __mul24(F[0], F[1]) + accF[2];
-                if (yf % 16 == 15) flush(accF, winF, slotF, yf);
+                if (yf % 16 == 15) flush(accF, winF, 0, 2, yf);
             }
             if (yg >= y0 && yg < y1) {
                 accG[0] = __mul24(GX[0], GX[0]) + accG[0];
@@ -299,16 +326,16 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, unsign
                 accG[3] = __mul24(GX[1], GX[1]) + accG[3];
                 accG[4] = __mul24(GY[1], GY[1]) + accG[4];
                 accG[5] = __mul24(GX[1], GY[1]) + accG[5];
-                if (yg % 16 == 15) flush(accG, winG, slotG, yg);
+                if (yg % 16 == 15) flush(accG, winG, 6, 1, yg);
             }
             if (ym >= y0 && ym < y1) {
+                const unsigned so = __builtin_amdgcn_readfirstlane(4u * (unsigned)ym * (unsigned)w);
 #pragma unroll
-                for (int p = 0; p < 2; p++)
-                    if (mid[p] && inside) ((ES_GLOBAL int32_t *)mid[p])[(size_t)ym * w + x] = M[p];
+                for (int p = 0; p < 2; p++) __builtin_amdgcn_raw_buffer_store_b32(M[p], mid[p], mo, so, 0);
                 accM[0] += (long long)M[0] * M[0];
                 accM[1] += (long long)M[1] * M[1];
                 accM[2] += (long long)M[0] * M[1];
-                if (ym % 16 == 15) flush(accM, winM, slotM, ym);
+                if (ym % 16 == 15) flush(accM, winM, 1, 2, ym);
             }
         }
     }
@@ -318,15 +345,19 @@ template <int WIN>
 __global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pairs)
 {
     __shared__ WsRows rowbuf;
+    __shared__ WsRed red;
     const uint3 b = xcd_block3();
     const WsPair P = pairs[b.z];  // a copy: the stores cannot change it, so it is read once
     // one reader per pair of plane formats, chosen once per block
     if (P.r.bytes == 1) {
-        if (P.d.bytes == 1) ws_strip<WIN, 1, 1>(P, rowbuf, b.x, b.y);
-        else ws_strip<WIN, 1, 2>(P, rowbuf, b.x, b.y);
+        if (P.d.bytes == 1) ws_strip<WIN, 1, 1, true>(P, rowbuf, red, b.x, b.y);
+        else ws_strip<WIN, 1, 2, true>(P, rowbuf, red, b.x, b.y);
+    } else if (P.d.bytes == 1) {
+        ws_strip<WIN, 2, 1, true>(P, rowbuf, red, b.x, b.y);
+    } else if ((P.r.rshift | P.r.shift | P.d.rshift | P.d.shift | P.up) == 0) {
+        ws_strip<WIN, 2, 2, false>(P, rowbuf, red, b.x, b.y);  // 10-bit words, the common case
     } else {
-        if (P.d.bytes == 1) ws_strip<WIN, 2, 1>(P, rowbuf, b.x, b.y);
-        else ws_strip<WIN, 2, 2>(P, rowbuf, b.x, b.y);
+        ws_strip<WIN, 2, 2, true>(P, rowbuf, red, b.x, b.y);
     }
 }
 
