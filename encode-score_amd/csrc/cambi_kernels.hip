@@ -584,14 +584,15 @@ __device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, unsigned long l
 typedef int cv_v4i __attribute__((ext_vector_type(4)));
 typedef int cv_v16i __attribute__((ext_vector_type(16)));
 constexpr int kMCnt = 144;  // bytes per number in the count table: 128 columns, rows 16-byte aligned on spread banks
-constexpr int kMWin = 68;   // u16 per number in the window table: 64 columns and 4 of spread
+constexpr int kMWin = 66;   // u16 per number in the window table: 64 columns and 2 of spread
+constexpr int kMChunks = (kMaxBand + 2 * kDiffs + 63) / 64;  // 64-value chunks of the relevance masks
 
 template <int NB>
 struct CvSharedM {
-    uint8_t cnt[32 * NB + 1][kMCnt];   // column counts of number n at margin column c; row B: the pass's discards
+    uint8_t cnt[32 * NB][kMCnt];       // column counts of number n at margin column c
     uint16_t win[32 * NB + 1][kMWin];  // window counts of number n at output column x; row B: zeros
-    uint16_t rank[kMaxBand + 2 * kDiffs];
     uint8_t row[kMaxBand + 2 * kDiffs];  // the pass's row of each value: its number less b0, or B outside the pass
+    unsigned long long relevant[kMChunks];  // bit i of chunk j: value 64 j + i is relevant (numbered in value order)
     __device__ uint8_t *flags() { return (uint8_t *)&win[0][0]; }
 };
 static_assert(sizeof(uint16_t) * 32 * kMWin >= 2 * kCvFlags, "flags live in the window table during the setup");
@@ -608,9 +609,9 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 }
 
 // Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds); the band of every output block
-// touches NJ margin blocks.  Per pass each value's row in the tables is looked up once (row[]), its number less b0
-// or B when it is outside the pass or not relevant: count updates of such samples go to the discard row B of cnt,
-// and window counts read from the zero row B of win, so the row loop makes no range tests.
+// touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[]), its number less b0, or B
+// when it is outside the pass or not relevant: window counts of such values read the zero row B of win, and their
+// count updates add 0 (at row B, past the count table, inside the window table).
 template <int NB, int NBU, int NJ>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
 {
@@ -634,13 +635,24 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
     const unsigned cword = (unsigned)(c0 >> 2) * 4u;
     const unsigned inc0 = 1u << ((c0 & 3) * 8), inc1 = 1u << (((c0 + 1) & 3) * 8);
     uint8_t *const cnt = &sh.cnt[0][0];
-    auto bump = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), inc); };
+    static_assert(sizeof(uint8_t) * 32 * NB * kMCnt + kMCnt <= sizeof(uint8_t) * 32 * NB * kMCnt + sizeof(sh.win),
+                  "the zero-adding updates at row B stay inside the window table");
+    auto bump = [&](unsigned row, unsigned inc) {
+        atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), row < (unsigned)B ? inc : 0u);
+    };
     const int nv = t.band_size + 2 * kDiffs;
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
-        for (int i = lane; i < (B + 1) * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
+        for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
         if (lane < kMWin / 2) ((uint32_t *)&sh.win[B][0])[lane] = 0u;
-        for (int i = lane; i < nv; i += 64) sh.row[i] = (uint8_t)min((unsigned)sh.rank[i] - (unsigned)b0, (unsigned)B);
+        // the rows of the pass: a relevant value's number is the count of relevant values below it
+        for (int j = 0, below = 0; j * 64 < nv; j++) {
+            const unsigned long long m = sh.relevant[j];
+            const unsigned n = (unsigned)below + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            const int i = j * 64 + lane;
+            if (i < nv) sh.row[i] = (uint8_t)((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B);
+            below += __popcll(m);
+        }
         __syncthreads();  // one wave: orders the table writes before the reads below
         const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
         for (int y = ya; y < yb; y += 8) {
@@ -705,15 +717,15 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                         for (int mb = 0; mb < 2; mb++)
                             acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc[mb], 0, 0, 0);
                     // D's register 4 q + i of this lane: output column 32 mb + 8 q + 4 (lane / 32) + i, number
-                    // 32 nb + lane % 32; four columns per 8-byte store
+                    // 32 nb + lane % 32; two columns per 4-byte store
 #pragma unroll
                     for (int mb = 0; mb < 2; mb++)
 #pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            const uint2 w2 = make_uint2((unsigned)acc[mb][4 * q] | (unsigned)acc[mb][4 * q + 1] << 16,
-                                                        (unsigned)acc[mb][4 * q + 2] | (unsigned)acc[mb][4 * q + 3] << 16);
-                            *(uint2 *)&sh.win[32 * nb + r32][32 * mb + 8 * q + 4 * g] = w2;
-                        }
+                        for (int q = 0; q < 4; q++)
+#pragma unroll
+                            for (int e = 0; e < 2; e++)
+                                *(uint32_t *)&sh.win[32 * nb + r32][32 * mb + 8 * q + 4 * g + 2 * e] =
+                                    (unsigned)acc[mb][4 * q + 2 * e] | (unsigned)acc[mb][4 * q + 2 * e + 1] << 16;
                 }
                 asm volatile("" ::: "memory");
                 unsigned cnt9[2 * kDiffs + 1];
@@ -788,6 +800,20 @@ __device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int r
     if (nj == 3) cv_run_m<NB, 3>(sh, t, relevant, sum_fx, tcount);
     else if (nj == 2) cv_run_m<NB, 2>(sh, t, relevant, sum_fx, tcount);
     else cv_run_m<NB, 1>(sh, t, relevant, sum_fx, tcount);
+}
+
+// The numbering of the relevant values of 64-value chunk i0 (m: which are relevant, `before` of them below it): a
+// rank per value for the prefix scans, the relevance mask for the matrix-core form (whose passes number from it).
+template <int BMAX>
+__device__ __forceinline__ void cv_number(CvShared<BMAX> &sh, int i0, bool valid, bool need, unsigned long long m, unsigned before)
+{
+    const int lane = threadIdx.x;
+    if (valid) sh.rank[i0 + lane] = need ? (uint16_t)(before + __popcll(m & ((1ull << lane) - 1))) : kNone;
+}
+template <int NB>
+__device__ __forceinline__ void cv_number(CvSharedM<NB> &sh, int i0, bool, bool, unsigned long long m, unsigned)
+{
+    if (threadIdx.x == 0) sh.relevant[i0 / 64] = m;
 }
 
 __device__ int find_slot(const int *first, int n, int t)
@@ -874,7 +900,7 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         if (i < nv && occupied[i])
             for (int k = -kDiffs; k <= kDiffs; k++) need |= i + k >= 0 && i + k < nv && served[i + k];
         const unsigned long long m = __ballot(need);
-        if (i < nv) sh.rank[i] = need ? (uint16_t)(relevant + __popcll(m & ((1ull << lane) - 1))) : kNone;
+        cv_number(sh, i0, i < nv, need, m, (unsigned)relevant);
         relevant += __popcll(m);
     }
     __syncthreads();
@@ -889,9 +915,14 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     }
 }
 
+// Waves per SIMD each layout is built for: four for the matrix-core form over 32 numbers (10 KB of LDS: 16 waves per
+// CU), three otherwise.
+template <typename Sh> constexpr int kCvWaves = 3;
+template <> constexpr int kCvWaves<CvSharedM<1>> = 4;
+
 // Sh: CvShared<BMAX> (prefix scans on the vector units) or CvSharedM<NB> (window counts on the matrix cores).
 template <typename Sh>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void cambi_cvalues_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCvWaves<Sh>))) void cambi_cvalues_kernel(
     const CambiPic *pics, const int *tile_first, int nslots, int tile_h, int persistent, unsigned int *counter)
 {
     __shared__ Sh sh;
