@@ -591,7 +591,10 @@ template <int NB>
 struct CvSharedM {
     uint8_t cnt[32 * NB][kMCnt];       // column counts of number n at margin column c
     uint16_t win[32 * NB + 1][kMWin];  // window counts of number n at output column x; row B: zeros
-    uint8_t row[kMaxBand + 2 * kDiffs];  // the pass's row of each value: its number less b0, or B outside the pass
+    // the pass's row of each value (its number less b0, or B outside the pass) at row[4 + i], i the value's index
+    // (value less band_base plus 4), with B at 0 .. 3 and from 4 + nv on, so that every index from 0 to nv and the 4
+    // values either side of it are entries
+    uint8_t row[kMaxBand + 6 * kDiffs + 4];
     unsigned long long relevant[kMChunks];  // bit i of chunk j: value 64 j + i is relevant (numbered in value order)
     __device__ uint8_t *flags() { return (uint8_t *)&win[0][0]; }
 };
@@ -611,7 +614,9 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 // Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds); the band of every output block
 // touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[]), its number less b0, or B
 // when it is outside the pass or not relevant: window counts of such values read the zero row B of win, and their
-// count updates add 0 (at row B, past the count table, inside the window table).
+// count updates add 0 (at row B, past the count table, inside the window table).  A packed sample's index is
+// min(sample - (0x8000 + band_base - 4), nv): value - band_base + 4 for a masked sample at or above band_base - 4 and
+// below the band's end, nv (or more, then clamped) otherwise; every index outside 4 .. nv - 5 has row B.
 template <int NB, int NBU, int NJ>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
 {
@@ -640,19 +645,23 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
     auto bump = [&](unsigned row, unsigned inc) {
         atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), row < (unsigned)B ? inc : 0u);
     };
-    const int nv = t.band_size + 2 * kDiffs;
+    const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs), ibase = 0x8000u + (unsigned)t.band_base - kDiffs;
+    auto index = [&](unsigned v) { return min(v - ibase, nv); };
+    const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the row of index i (i from -4 to nv + 4)
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
         for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
         if (lane < kMWin / 2) ((uint32_t *)&sh.win[B][0])[lane] = 0u;
         // the rows of the pass: a relevant value's number is the count of relevant values below it
-        for (int j = 0, below = 0; j * 64 < nv; j++) {
+        for (int j = 0, below = 0; j * 64 < (int)nv; j++) {
             const unsigned long long m = sh.relevant[j];
             const unsigned n = (unsigned)below + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
             const int i = j * 64 + lane;
-            if (i < nv) sh.row[i] = (uint8_t)((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B);
+            if (i < (int)nv) sh.row[kDiffs + i] = (uint8_t)((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B);
             below += __popcll(m);
         }
+        if (lane < kDiffs) sh.row[lane] = (uint8_t)B;
+        if (lane < 2 * kDiffs + 1) sh.row[kDiffs + nv + lane] = (uint8_t)B;
         __syncthreads();  // one wave: orders the table writes before the reads below
         const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
         for (int y = ya; y < yb; y += 8) {
@@ -665,13 +674,16 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
             }
             unsigned rw[16];
 #pragma unroll
-            for (int j = 0; j < 16; j++) rw[j] = sh.row[cv_index(t, v[j])];
+            for (int j = 0; j < 16; j++) rw[j] = rows[index(v[j])];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 bump(rw[2 * j], inc0);
                 bump(rw[2 * j + 1], inc1);
             }
         }
+        // codes of a row: entering (y + pad), own (y), leaving (y - pad); loaded a row ahead into the other of two
+        // arrays (the row loop is unrolled twice, so they swap roles by renaming).  The leaving samples of row y are
+        // removed at the start of row y + 1, with that row's entering ones.
         auto load = [&](int y, unsigned *v) {
             v[0] = cv_at(t, xo0, y + pad);
             v[1] = cv_at(t, xo1, y + pad);
@@ -679,23 +691,21 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
             v[3] = cv_at(t, xo0, y - pad);
             v[4] = cv_at(t, xo1, y - pad);
         };
-        unsigned cur[5], nxt[5] = {0, 0, 0, 0, 0}, gone0 = 0, gone1 = 0;
-        load(t.y0, cur);
         const unsigned lo_rel = (unsigned)(lo - b0), hi_rel = (unsigned)(hi - b0);
-        for (int r = 0; r < t.rows; r++) {
+        auto step = [&](const int r, const unsigned *cur, unsigned *nxt, unsigned gone0, unsigned gone1) __attribute__((always_inline)) {
             const int y = t.y0 + r;
-            if (r + 1 < t.rows) load(y + 1, nxt);
-            const unsigned io = cv_index(t, cur[2]), base = io ? io - kDiffs : 0u;
-            const unsigned re0 = sh.row[cv_index(t, cur[0])], re1 = sh.row[cv_index(t, cur[1])];
-            const unsigned rl0 = sh.row[cv_index(t, gone0)], rl1 = sh.row[cv_index(t, gone1)];
-            unsigned rk[2 * kDiffs + 1];
+            const unsigned io = index(cur[2]);
+            const unsigned re0 = rows[index(cur[0])], re1 = rows[index(cur[1])];
+            const unsigned rl0 = rows[index(gone0)], rl1 = rows[index(gone1)];
+            unsigned rk[2 * kDiffs + 1];  // rows of the own value and the 4 values either side
 #pragma unroll
-            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = sh.row[base + k];
+            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = rows[io - kDiffs + k];
+            if (r + 1 < t.rows) load(y + 1, nxt);
             bump(rl0, 0u - inc0);
             bump(rl1, 0u - inc1);
             bump(re0, inc0);
             bump(re1, inc1);
-            const unsigned own = io ? rk[kDiffs] : (unsigned)B;
+            const unsigned own = rk[kDiffs];
             const bool serve = own >= lo_rel && own < hi_rel;
             if (__builtin_amdgcn_ballot_w64(serve)) {
                 // the LDS operations of one wave complete in order: the row's updates are in before these reads
@@ -731,7 +741,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                 unsigned cnt9[2 * kDiffs + 1];
 #pragma unroll
                 for (int k = 0; k < 2 * kDiffs + 1; k++) cnt9[k] = sh.win[rk[k]][lane];
-                const int value = (int)(io ? io - kDiffs : 0u) + t.band_base + kDiffs;
+                const int value = (int)io + t.band_base;  // on a served pixel: its value + 4, as the thresholds take it
                 const unsigned p0 = cnt9[kDiffs];
                 float c = 0.0f;
 #pragma unroll
@@ -759,10 +769,15 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                 tcount += (unsigned)__popcll(m);
                 asm volatile("" ::: "memory");  // every read of the table before the next row writes it
             }
-            gone0 = cur[3];
-            gone1 = cur[4];
-#pragma unroll
-            for (int k = 0; k < 5; k++) cur[k] = nxt[k];
+        };
+        unsigned ca[5], cb[5] = {0, 0, 0, 0, 0};
+        load(t.y0, ca);
+        // the first row removes nothing: index(0) is nv, whose row is B
+        step(0, ca, cb, 0u, 0u);
+        for (int r = 1; r < t.rows; r += 2) {
+            step(r, cb, ca, ca[3], ca[4]);
+            if (r + 1 >= t.rows) break;
+            step(r + 1, ca, cb, cb[3], cb[4]);
         }
         if (hi == relevant) break;
         b0 = hi - kDiffs;
@@ -847,7 +862,7 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
 #pragma unroll
     for (int d = 0; d < kDiffs; d++) {
         t.tvi[d] = __builtin_amdgcn_readfirstlane(K.tvi[d]);
-        t.weight[d] = __builtin_amdgcn_readfirstlane(K.weight[d]);
+        t.weight[d] = __builtin_amdgcn_readfirstlane(K.weight[d]) & 0xFF;  // 1 .. 4: 24-bit products stay exact
     }
     t.plane = pic.cvals ? (ES_GLOBAL float *)(pic.cvals + pic.off[s]) : nullptr;
     t.seg = (ES_GLOBAL float *)(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h);
