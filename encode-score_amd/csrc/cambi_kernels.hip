@@ -589,7 +589,7 @@ constexpr int kMChunks = (kMaxBand + 2 * kDiffs + 63) / 64;  // 64-value chunks 
 
 template <int NB>
 struct CvSharedM {
-    uint8_t cnt[32 * NB][kMCnt];       // column counts of number n at margin column c
+    alignas(16) uint8_t cnt[32 * NB][kMCnt];  // column counts of number n at margin column c
     uint16_t win[32 * NB + 1][kMWin];  // window counts of number n at output column x; row B: zeros
     // the pass's row of each value (its number less b0, or B outside the pass) at row[4 + i], i the value's index
     // (value less band_base plus 4), with B at 0 .. 3 and from 4 + nv on, so that every index from 0 to nv and the 4
@@ -614,7 +614,8 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 // Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds); the band of every output block
 // touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[]), its number less b0, or B
 // when it is outside the pass or not relevant: window counts of such values read the zero row B of win, and their
-// count updates add 0 (at row B, past the count table, inside the window table).  A packed sample's index is
+// count updates go to row B of cnt, which is unused or (past the count table) the first 128 bytes of win's row 0,
+// which every served row rewrites in full before it reads any of it.  A packed sample's index is
 // min(sample - (0x8000 + band_base - 4), nv): value - band_base + 4 for a masked sample at or above band_base - 4 and
 // below the band's end, nv (or more, then clamped) otherwise; every index outside 4 .. nv - 5 has row B.
 template <int NB, int NBU, int NJ>
@@ -640,11 +641,13 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
     const unsigned cword = (unsigned)(c0 >> 2) * 4u;
     const unsigned inc0 = 1u << ((c0 & 3) * 8), inc1 = 1u << (((c0 + 1) & 3) * 8);
     uint8_t *const cnt = &sh.cnt[0][0];
-    static_assert(sizeof(uint8_t) * 32 * NB * kMCnt + kMCnt <= sizeof(uint8_t) * 32 * NB * kMCnt + sizeof(sh.win),
-                  "the zero-adding updates at row B stay inside the window table");
-    auto bump = [&](unsigned row, unsigned inc) {
-        atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), row < (unsigned)B ? inc : 0u);
-    };
+    static_assert(offsetof(CvSharedM<NB>, cnt) + 32 * NB * kMCnt == offsetof(CvSharedM<NB>, win) && kMWin * 2 >= 128,
+                  "updates at row B past the count table land in win's row 0");
+    auto bump = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), inc); };
+    // this lane's start in the count table (B operand: number r32, its 16 margin columns of a block) and in the
+    // window table (the products it writes: number r32, output columns 4 g ..)
+    const uint8_t *const crow = &sh.cnt[r32][16 * g];
+    uint16_t *const wrow = &sh.win[r32][4 * g];
     const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs), ibase = 0x8000u + (unsigned)t.band_base - kDiffs;
     auto index = [&](unsigned v) { return min(v - ibase, nv); };
     const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the row of index i (i from -4 to nv + 4)
@@ -700,7 +703,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
             unsigned rk[2 * kDiffs + 1];  // rows of the own value and the 4 values either side
 #pragma unroll
             for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = rows[io - kDiffs + k];
-            if (r + 1 < t.rows) load(y + 1, nxt);
+            load(y + 1, nxt);  // past the tile's last row it reads unused samples (past the picture, zeros)
             bump(rl0, 0u - inc0);
             bump(rl1, 0u - inc1);
             bump(re0, inc0);
@@ -714,7 +717,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
 #pragma unroll
                 for (int nb = 0; nb < NBU; nb++)
 #pragma unroll
-                    for (int kb = 0; kb <= NJ; kb++) bop[nb][kb] = *(const cv_v4i *)&sh.cnt[32 * nb + r32][32 * kb + 16 * g];
+                    for (int kb = 0; kb <= NJ; kb++) bop[nb][kb] = *(const cv_v4i *)(crow + (32 * nb * kMCnt + 32 * kb));
 #pragma unroll
                 for (int nb = 0; nb < NBU; nb++) {
                     // the two output blocks' products interleaved, so that neither waits on its own previous one
@@ -734,7 +737,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                         for (int q = 0; q < 4; q++)
 #pragma unroll
                             for (int e = 0; e < 2; e++)
-                                *(uint32_t *)&sh.win[32 * nb + r32][32 * mb + 8 * q + 4 * g + 2 * e] =
+                                *(uint32_t *)(wrow + (32 * nb * kMWin + 32 * mb + 8 * q + 2 * e)) =
                                     (unsigned)acc[mb][4 * q + 2 * e] | (unsigned)acc[mb][4 * q + 2 * e + 1] << 16;
                 }
                 asm volatile("" ::: "memory");
@@ -749,11 +752,13 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                     const unsigned pa = cnt9[kDiffs + d + 1], pb = cnt9[kDiffs - d - 1];
                     const unsigned pm = pa > pb ? pa : pb;
                     // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; computed on
-                    // every lane and selected (p0 + pm is at least 1 on a served pixel, its own count)
+                    // every lane and selected.  On a served pixel p0 + pm is at least 1 (its own count) and pm = 0
+                    // gives 0, which never exceeds c, so the reference's pm > 0 test needs no instruction; elsewhere a
+                    // NaN (0 / 0) exceeds nothing either
                     const float rcp = recip_rn(p0 + pm);
                     float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
                     asm("" : "+v"(val));
-                    const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt && pm > 0;
+                    const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt;
                     c = on && val > c ? val : c;
                 }
                 c = serve ? c : 0.0f;
