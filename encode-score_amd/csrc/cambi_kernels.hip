@@ -331,6 +331,7 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int kCvMaxB = 64;  // the largest pass; kernels are built for 32 and 64
 constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
+constexpr unsigned kCvDummy = kCvFlags - 1;  // the flag an inactive sample sets: past every index a tile reads
 constexpr int kCvCols = 130;    // prefix entries per number: columns 0 .. 128, padded to an even count
 
 // LDS of one wave for passes of up to BMAX numbers: 14.5 KB for 32, 26.7 KB for 64 (fewer passes over tiles with
@@ -822,18 +823,25 @@ __device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int r
     else cv_run_m<NB, 1>(sh, t, relevant, sum_fx, tcount);
 }
 
-// The numbering of the relevant values of 64-value chunk i0 (m: which are relevant, `before` of them below it): a
-// rank per value for the prefix scans, the relevance mask for the matrix-core form (whose passes number from it).
+// The numbering of the relevant values: lane c holds chunk c's relevance mask (values 64 c .. 64 c + 63) and the
+// number of relevant values below the chunk.  The prefix scans keep a rank per value; the matrix-core form keeps the
+// masks, from which each pass numbers.
 template <int BMAX>
-__device__ __forceinline__ void cv_number(CvShared<BMAX> &sh, int i0, bool valid, bool need, unsigned long long m, unsigned before)
+__device__ __forceinline__ void cv_number(CvShared<BMAX> &sh, unsigned long long need, unsigned before, int nchunks, int nv)
 {
     const int lane = threadIdx.x;
-    if (valid) sh.rank[i0 + lane] = need ? (uint16_t)(before + __popcll(m & ((1ull << lane) - 1))) : kNone;
+    for (int c = 0; c < nchunks; c++) {
+        const unsigned long long m = (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)need, c) |
+                                     (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(need >> 32), c) << 32;
+        const unsigned b = (unsigned)__builtin_amdgcn_readlane((int)before, c);
+        const int i = 64 * c + lane;
+        if (i < nv) sh.rank[i] = (m >> lane & 1ull) ? (uint16_t)(b + __popcll(m & ((1ull << lane) - 1))) : kNone;
+    }
 }
 template <int NB>
-__device__ __forceinline__ void cv_number(CvSharedM<NB> &sh, int i0, bool, bool, unsigned long long m, unsigned)
+__device__ __forceinline__ void cv_number(CvSharedM<NB> &sh, unsigned long long need, unsigned, int nchunks, int)
 {
-    if (threadIdx.x == 0) sh.relevant[i0 / 64] = m;
+    if ((int)threadIdx.x < nchunks) sh.relevant[threadIdx.x] = need;
 }
 
 __device__ int find_slot(const int *first, int n, int t)
@@ -876,23 +884,25 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
     const int lane = threadIdx.x, nv = t.band_size + 2 * kDiffs;
     const int c0 = 2 * lane, xl = t.x0 - t.pad + c0, cw = kCvTileW + 2 * t.pad;
-    __syncthreads();  // the previous tile's reads of pp and rank are done
-    uint8_t *served = cv_flags(sh), *occupied = served + kCvFlags;
-    for (int i = lane; i < 2 * kCvFlags / 4; i += 64) ((uint32_t *)served)[i] = 0;
+    __syncthreads();  // the previous tile's reads of the tables are done
+    // the served values (of the tile's masked in-band pixels) and the occupied ones (of the tile and its margins) as
+    // byte flags over value indices (value - band_base + 4); an inactive sample sets the unread flag kCvDummy
+    uint8_t *const served = cv_flags(sh), *const occupied = served + kCvFlags;
+    for (int i = lane; i < 2 * kCvFlags / 4; i += 64) ((uint32_t *)served)[i] = 0u;
     __syncthreads();
+    auto flag = [&](uint8_t *flags, unsigned v) {
+        const unsigned u = cv_off(t, v);
+        const bool active = u < (unsigned)t.band_size;
+        flags[active ? u + kDiffs : kCvDummy] = 1;
+        return active;
+    };
     bool any = false;
     for (int r = 0; r < t.rows; r += 8) {
         unsigned v[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) v[j] = cv_at(t, cv_col(t, t.x0 + lane), r + j < t.rows ? t.y0 + r + j : -1);
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const unsigned u = cv_decode(t, v[j]);
-            if (u != kNone) {
-                served[u + kDiffs] = 1;
-                any = true;
-            }
-        }
+        for (int j = 0; j < 8; j++) any |= flag(served, v[j]);
     }
     if (!__any(any)) {  // most tiles of most pictures: nothing masked inside the band
         if (lane == 0) *seg_count = 0;
@@ -907,22 +917,30 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
             v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
         }
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            const unsigned u = cv_decode(t, v[j]);
-            if (u != kNone) occupied[u + kDiffs] = 1;
-        }
+        for (int j = 0; j < 16; j++) flag(occupied, v[j]);
     }
     __syncthreads();
-    int relevant = 0;
-    for (int i0 = 0; i0 < nv; i0 += 64) {
-        const int i = i0 + lane;
-        bool need = false;
-        if (i < nv && occupied[i])
-            for (int k = -kDiffs; k <= kDiffs; k++) need |= i + k >= 0 && i + k < nv && served[i + k];
-        const unsigned long long m = __ballot(need);
-        cv_number(sh, i0, i < nv, need, m, (unsigned)relevant);
-        relevant += __popcll(m);
+    // the relevant values, 64 per chunk, lane c holding chunk c: occupied and within 4 of a served value (the served
+    // set widened by 4 either way, across chunk edges), below nv; numbered in value order
+    const int nchunks = (nv + 63) / 64;
+    unsigned long long sc = 0, oc = 0;  // this lane's chunk of the two sets
+    for (int c = 0; c < nchunks; c++) {
+        const int i = 64 * c + lane;
+        const unsigned long long sm = __builtin_amdgcn_ballot_w64(i < nv && served[i]);
+        const unsigned long long om = __builtin_amdgcn_ballot_w64(i < nv && occupied[i]);
+        if (lane == c) {
+            sc = sm;
+            oc = om;
+        }
     }
+    const unsigned long long sp = __shfl_up(sc, 1), sn = __shfl_down(sc, 1);
+    unsigned long long near = sc;
+#pragma unroll
+    for (int m = 1; m <= kDiffs; m++) near |= sc >> m | (lane + 1 < nchunks ? sn : 0ull) << (64 - m) | sc << m | (lane ? sp : 0ull) >> (64 - m);
+    const unsigned long long need = lane < nchunks ? oc & near : 0ull;
+    const unsigned count = (unsigned)__popcll(need), after = wave_scan_incl(count);
+    const int relevant = (int)__builtin_amdgcn_readlane(after, 63);
+    cv_number(sh, need, after - count, nchunks, nv);
     __syncthreads();
     unsigned long long sum_fx = 0;
     unsigned tcount = 0;
