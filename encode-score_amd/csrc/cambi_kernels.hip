@@ -329,7 +329,6 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // processed, the setup's loads go out eight rows at a time, and the c-value arithmetic is predicated rather than
 // branched.  Positive c-values go to the tile's own segment of the list at a wave-uniform offset.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int kCvMaxB = 64;  // the largest pass; kernels are built for 32 and 64
 constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
 constexpr unsigned kCvDummy = kCvFlags - 1;  // the flag an inactive sample sets: past every index a tile reads
 constexpr int kCvCols = 130;    // prefix entries per number: columns 0 .. 128, padded to an even count
@@ -353,9 +352,8 @@ struct CvTile {
     int w, h, x0, y0, rows;
     int pad, band_base, band_size, vlt;
     int tvi[kDiffs], weight[kDiffs];
-    ES_GLOBAL float *plane;  // full c-value plane or nullptr
-    ES_GLOBAL float *seg;    // this tile's segment of the list
-    __amdgpu_buffer_rsrc_t plane_rs, seg_rs;  // the same as buffers (no plane: an empty one)
+    // the full c-value plane (when kept; else an empty buffer) and this tile's segment of the list, as buffers
+    __amdgpu_buffer_rsrc_t plane_rs, seg_rs;
 };
 
 // Samples are read as buffer loads at byte offset xo + yo: a load past the buffer's w * h samples returns 0, which is
@@ -542,20 +540,24 @@ __device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, unsigned long l
                 for (int d = 0; d < kDiffs; d++) {
                     const unsigned pa = cnt[kDiffs + d + 1], pb = cnt[kDiffs - d - 1];
                     const unsigned pm = pa > pb ? pa : pb;
-                    // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; p0 + pm is
-                    // at least 1 on a served pixel (its own count), and other lanes' results are discarded
+                    // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; computed on
+                    // every lane and selected, the tests combined without short-circuiting.  On a served pixel
+                    // p0 + pm is at least 1 (its own count) and pm = 0 gives 0, which never exceeds c, so the
+                    // reference's pm > 0 test needs no instruction; elsewhere a NaN (0 / 0) exceeds nothing either
                     const float rcp = recip_rn(p0 + pm);
-                    const float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
-                    const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt && pm > 0;
-                    c = on && val > c ? val : c;
+                    float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
+                    asm("" : "+v"(val));
+                    const bool take = (value <= t.tvi[d]) & (value + d + 1 > t.vlt) & (val > c);
+                    c = take ? val : c;
                 }
                 c = serve ? c : 0.0f;
-                if (t.plane) {
-                    if (serve) t.plane[(unsigned)(y * t.w + x)] = c;
-                }
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.plane_rs, serve ? 4u * (unsigned)x : 0x80000000u,
+                                                      __builtin_amdgcn_readfirstlane(4u * (unsigned)y * (unsigned)t.w), 0);
                 sum_fx += c > 0.0f ? fixed24(c) : 0ull;
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
-                if (c > 0.0f) t.seg[tcount + __popcll(m & ((1ull << lane) - 1))] = c;
+                const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.seg_rs, c > 0.0f ? 4u * below : 0x80000000u,
+                                                      __builtin_amdgcn_readfirstlane(4u * tcount), 0);
                 tcount += (unsigned)__popcll(m);
                 __syncthreads();  // every lane has read the prefixes before the next row writes them
             }
@@ -759,8 +761,9 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                     const float rcp = recip_rn(p0 + pm);
                     float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
                     asm("" : "+v"(val));
-                    const bool on = value <= t.tvi[d] && value + d + 1 > t.vlt;
-                    c = on && val > c ? val : c;
+                    // the tests combined without short-circuiting, so that they are compares and one select
+                    const bool take = (value <= t.tvi[d]) & (value + d + 1 > t.vlt) & (val > c);
+                    c = take ? val : c;
                 }
                 c = serve ? c : 0.0f;
                 // the full plane (when kept) and the tile's list, as buffer stores: a lane with nothing to store
@@ -877,8 +880,6 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         t.tvi[d] = __builtin_amdgcn_readfirstlane(K.tvi[d]);
         t.weight[d] = __builtin_amdgcn_readfirstlane(K.weight[d]) & 0xFF;  // 1 .. 4: 24-bit products stay exact
     }
-    t.plane = pic.cvals ? (ES_GLOBAL float *)(pic.cvals + pic.off[s]) : nullptr;
-    t.seg = (ES_GLOBAL float *)(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h);
     t.plane_rs = buffer_over(pic.cvals ? pic.cvals + pic.off[s] : nullptr, pic.cvals ? (unsigned long long)t.w * t.h * 4 : 0);
     t.seg_rs = buffer_over(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h, (unsigned long long)kCvTileW * tile_h * 4);
     unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
