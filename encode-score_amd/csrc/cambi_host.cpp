@@ -11,6 +11,7 @@ static constexpr int kTopkBlocks = 64;
 CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues, int tile_h)
     : info_(cambi_info(w, h, depth, window_base)), keep_(keep_cvalues), tile_h_(tile_h > 0 ? tile_h : kDefaultTileH)
 {
+    if (tile_h_ > kMaxTileH) throw Error(ES_EINVAL, "tile_h must be at most 4096");
     ensure_init();
     const size_t total = (size_t)info_.off[kScales];
     CambiConsts k{};

@@ -400,7 +400,7 @@ __device__ __forceinline__ unsigned cv_decode(const CvTile &t, unsigned v)
 }
 
 template <int B, typename Sh>
-__device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
+__device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, double &sum_c, unsigned &tcount)
 {
     constexpr int Q = B / 4, NC = B / 8;  // count words per column; chunks of 8 numbers (2 words, 4 pairs)
     const int lane = threadIdx.x, pad = t.pad, span = 2 * pad + 1;
@@ -553,7 +553,7 @@ __device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, unsigned long l
                 c = serve ? c : 0.0f;
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.plane_rs, serve ? 4u * (unsigned)x : 0x80000000u,
                                                       __builtin_amdgcn_readfirstlane(4u * (unsigned)y * (unsigned)t.w), 0);
-                sum_fx += c > 0.0f ? fixed24(c) : 0ull;
+                sum_c += (double)c;  // exact: see cv_tile
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
                 const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.seg_rs, c > 0.0f ? 4u * below : 0x80000000u,
@@ -622,7 +622,7 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 // min(sample - (0x8000 + band_base - 4), nv): value - band_base + 4 for a masked sample at or above band_base - 4 and
 // below the band's end, nv (or more, then clamped) otherwise; every index outside 4 .. nv - 5 has row B.
 template <int NB, int NBU, int NJ>
-__device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx, unsigned &tcount)
+__device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c, unsigned &tcount)
 {
     static_assert(NBU >= 1 && NBU <= NB && NJ >= 1 && NJ <= 3, "blocks of numbers, band blocks");
     constexpr int B = 32 * NBU;
@@ -770,7 +770,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, un
                 // stores past the end, where the store is dropped
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.plane_rs, serve ? 4u * (unsigned)x : 0x80000000u,
                                                       __builtin_amdgcn_readfirstlane(4u * (unsigned)y * (unsigned)t.w), 0);
-                sum_fx += c > 0.0f ? fixed24(c) : 0ull;
+                sum_c += (double)c;  // exact: see cv_tile
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
                 const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.seg_rs, c > 0.0f ? 4u * below : 0x80000000u,
@@ -800,30 +800,30 @@ template <int NB>
 __device__ __forceinline__ uint8_t *cv_flags(CvSharedM<NB> &sh) { return sh.flags(); }
 
 template <int BMAX>
-__device__ __forceinline__ void cv_run(CvShared<BMAX> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
+__device__ __forceinline__ void cv_run(CvShared<BMAX> &sh, const CvTile &t, int relevant, double &sum_c,
                                        unsigned &tcount)
 {
-    if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_fx, tcount);
-    else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_fx, tcount);
-    else if (BMAX == 32 || relevant <= 32) cv_passes<32>(sh, t, relevant, sum_fx, tcount);
-    else cv_passes<(BMAX > 32 ? BMAX : 32)>(sh, t, relevant, sum_fx, tcount);
+    if (relevant <= 8) cv_passes<8>(sh, t, relevant, sum_c, tcount);
+    else if (relevant <= 16) cv_passes<16>(sh, t, relevant, sum_c, tcount);
+    else if (BMAX == 32 || relevant <= 32) cv_passes<32>(sh, t, relevant, sum_c, tcount);
+    else cv_passes<(BMAX > 32 ? BMAX : 32)>(sh, t, relevant, sum_c, tcount);
 }
 template <int NB, int NJ>
-__device__ __forceinline__ void cv_run_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
+__device__ __forceinline__ void cv_run_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c,
                                          unsigned &tcount)
 {
-    if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ>(sh, t, relevant, sum_fx, tcount);
-    else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ>(sh, t, relevant, sum_fx, tcount);
+    if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ>(sh, t, relevant, sum_c, tcount);
+    else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ>(sh, t, relevant, sum_c, tcount);
 }
 template <int NB>
-__device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int relevant, unsigned long long &sum_fx,
+__device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c,
                                        unsigned &tcount)
 {
     // the margin blocks the band of an output block touches: 2 pad + 1 columns from the block's first
     const int nj = min((31 + 2 * t.pad) / 32 + 1, 3);
-    if (nj == 3) cv_run_m<NB, 3>(sh, t, relevant, sum_fx, tcount);
-    else if (nj == 2) cv_run_m<NB, 2>(sh, t, relevant, sum_fx, tcount);
-    else cv_run_m<NB, 1>(sh, t, relevant, sum_fx, tcount);
+    if (nj == 3) cv_run_m<NB, 3>(sh, t, relevant, sum_c, tcount);
+    else if (nj == 2) cv_run_m<NB, 2>(sh, t, relevant, sum_c, tcount);
+    else cv_run_m<NB, 1>(sh, t, relevant, sum_c, tcount);
 }
 
 // The numbering of the relevant values: lane c holds chunk c's relevance mask (values 64 c .. 64 c + 63) and the
@@ -943,10 +943,14 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     const int relevant = (int)__builtin_amdgcn_readlane(after, 63);
     cv_number(sh, need, after - count, nchunks, nv);
     __syncthreads();
-    unsigned long long sum_fx = 0;
+    // A lane's positive c-values are at least 0.5, so multiples of 2^-24, each at most 4 * 65^2 / 2 < 2^14, and it serves
+    // at most one pixel per row of the tile (at most kMaxTileH = 2^12 rows): its double sum stays below 2^26 in whole
+    // units of 2^-24, so it is exact, and so is the product by 2^24 below (an integer under 2^50), the c-values' exact
+    // sum in units of 2^-24.
+    double sum_c = 0.0;
     unsigned tcount = 0;
-    if (relevant > 0) cv_run(sh, t, relevant, sum_fx, tcount);
-    sum_fx = wave_sum(sum_fx);
+    if (relevant > 0) cv_run(sh, t, relevant, sum_c, tcount);
+    const unsigned long long sum_fx = wave_sum((unsigned long long)(sum_c * 16777216.0));
     if (lane == 0) {
         *seg_count = tcount;
         if (tcount) atomicAdd(pic.counts + s, tcount);

@@ -18,6 +18,7 @@ constexpr int kMaxPad = 32;      // one wave covers the tile and both margins as
 constexpr int kMaxBand = 1024;   // band values, 10-bit codes
 constexpr int kHistBins = 20001; // per-window histograms: 0.05-point bins over [-500, 500]
 constexpr int kRing = 13, kRadius = 6;
+constexpr int kMaxTileH = 4096;  // c-value tile heights: a lane's exact double sum of a tile's c-values needs < 2^20
 
 // Thrown by every host function on a HIP error or a bad argument; the C ABI turns it into a status code.
 struct Error : std::runtime_error {

@@ -444,6 +444,7 @@ public:
         require(B <= 64, "batch must be at most 64 frames");
         R = B + 16;
         tile_h = c.tile_h > 0 ? c.tile_h : 32;
+        require(tile_h <= kMaxTileH, "tile_h must be at most 4096");
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "device attributes");
         cv_blocks = std::max(1, cus) * (c.cv_waves > 0 ? c.cv_waves : 16);

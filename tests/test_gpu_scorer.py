@@ -181,7 +181,7 @@ def test_batched_larger_windows_match_oracle(gpu, orc, win):
 
 @pytest.mark.parametrize("mfma", [False, True])
 @pytest.mark.parametrize("bmax,tile_h,waves", [(64, 32, 16), (64, 16, 8), (32, 48, 24), (32, 64, 12), (32, 32, 0),
-                                               (64, 24, 0)])
+                                               (64, 24, 0), (32, 360, 16)])
 def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves, mfma):
     """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, waves per CU or
     one block per tile (waves 0)) gives the oracle's results; textured 640x360 frames have tiles with more than 32
@@ -202,6 +202,12 @@ def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves, mf
     ref = orc.accum_summary(run.segment(0))
     compare_segment(got, ref)
     assert np.array_equal(hist, ref[1])
+
+
+def test_tile_height_is_bounded(gpu):
+    # a lane sums a tile's c-values exactly in FP64 for tiles of at most 4096 rows
+    with pytest.raises(gpu.EsError, match="tile_h"):
+        gpu.Scorer(gpu.ES_BATCHED, 64, 64, [gpu.EncodeDesc(64, 64, 4, 64, 2, 0)], batch=4, tile_h=4097)
 
 
 def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
