@@ -219,12 +219,16 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
     const __amdgpu_buffer_rsrc_t mid[2] = {buffer_over(P.mid_r, P.mid_r ? (unsigned long long)w * h * 4 : 0),
                                            buffer_over(P.mid_d, P.mid_d ? (unsigned long long)w * h * 4 : 0)};
     const unsigned mo = inside ? 4u * (unsigned)x : 0x80000000u;
-    int pr[2], ph[2];
+    // samples of rows r .. r + 2 in slots (r - y0 + 4) mod 3: a row is loaded three rows before it is used (the row
+    // loop is unrolled 9 times, so the slots rotate by renaming)
+    int pr[3][2], ph[3][2];
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
-        pr[p] = sample(p, ca[p], y0 - 4);
-        ph[p] = sample(p, ch[p], y0 - 4);
-    }
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            pr[k][p] = sample(p, ca[p], y0 - 4 + k);
+            ph[k][p] = sample(p, ch[p], y0 - 4 + k);
+        }
     int h3c[2][9], h3l[2][3], h3r[2][3], h9[2][9], yv[2][3], s9[2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
@@ -275,13 +279,13 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
             const int r = base + i;
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                rowbuf[buf][p][lane + 4] = pr[p];
-                rowbuf[buf][p][hslot] = ph[p];
+                rowbuf[buf][p][lane + 4] = pr[i % 3][p];
+                rowbuf[buf][p][hslot] = ph[i % 3][p];
             }
 #pragma unroll
             for (int p = 0; p < 2; p++) {
-                pr[p] = sample(p, ca[p], r + 1);
-                ph[p] = sample(p, ch[p], r + 1);
+                pr[i % 3][p] = sample(p, ca[p], r + 3);
+                ph[i % 3][p] = sample(p, ch[p], r + 3);
             }
             __syncthreads();  // the row is in (one wave per block: no other wave waits here)
             int F[2], GX[2], GY[2], M[2];
@@ -342,7 +346,7 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
 }
 
 template <int WIN>
-__global__ __launch_bounds__(64) void window_stats_int_kernel(const WsPair *pairs)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void window_stats_int_kernel(const WsPair *pairs)
 {
     __shared__ WsRows rowbuf;
     __shared__ WsRed red;
