@@ -64,13 +64,13 @@ struct BaseRead16 {  // 16-bit words holding the samples as they are (10-bit pla
     __amdgpu_buffer_rsrc_t rs;
     int w, h;
     unsigned step;
-    __device__ unsigned col(int x) const { return 2u * (unsigned)min(max(x, 0), w - 1); }
-    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
-    __device__ int load(unsigned c, unsigned r) const { return (int)__builtin_amdgcn_raw_buffer_load_b16(rs, c, r, 0); }
+    __device__ __forceinline__ unsigned col(int x) const { return 2u * (unsigned)min(max(x, 0), w - 1); }
+    __device__ __forceinline__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
+    __device__ __forceinline__ int load(unsigned c, unsigned r) const { return (int)__builtin_amdgcn_raw_buffer_load_b16(rs, c, r, 0); }
 };
 struct BaseRead16S : BaseRead16 {  // 16-bit words shifted: (word >> rshift) << shift (P010's high bits)
     int rshift, shift;
-    __device__ int load(unsigned c, unsigned r) const
+    __device__ __forceinline__ int load(unsigned c, unsigned r) const
     {
         return (int)(((unsigned)__builtin_amdgcn_raw_buffer_load_b16(rs, c, r, 0) >> rshift) << shift);
     }
@@ -80,17 +80,17 @@ struct BaseRead8 {  // bytes, shifted to the pipeline depth
     int w, h;
     unsigned step;
     int shift;
-    __device__ unsigned col(int x) const { return (unsigned)min(max(x, 0), w - 1); }
-    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
-    __device__ int load(unsigned c, unsigned r) const { return (int)((unsigned)__builtin_amdgcn_raw_buffer_load_b8(rs, c, r, 0) << shift); }
+    __device__ __forceinline__ unsigned col(int x) const { return (unsigned)min(max(x, 0), w - 1); }
+    __device__ __forceinline__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1) * step; }
+    __device__ __forceinline__ int load(unsigned c, unsigned r) const { return (int)((unsigned)__builtin_amdgcn_raw_buffer_load_b8(rs, c, r, 0) << shift); }
 };
 struct BaseReadFrom8 {  // cambi.c's 8-bit anti-dithering mean; offsets are the clamped coordinates
     PlaneIn in;
     int w, h;
     unsigned step;
-    __device__ unsigned col(int x) const { return (unsigned)min(max(x, 0), w - 1); }
-    __device__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1); }
-    __device__ int load(unsigned c, unsigned r) const { return (int)base_value(in, w, h, (int)c, (int)r); }
+    __device__ __forceinline__ unsigned col(int x) const { return (unsigned)min(max(x, 0), w - 1); }
+    __device__ __forceinline__ unsigned row(int y) const { return (unsigned)min(max(y, 0), h - 1); }
+    __device__ __forceinline__ int load(unsigned c, unsigned r) const { return (int)base_value(in, w, h, (int)c, (int)r); }
 };
 
 // One strip.  Positions outside the picture enter only as wave masks: margin columns outside it and rows outside it
