@@ -58,6 +58,30 @@ __device__ __forceinline__ uint32_t wave_scan_incl(uint32_t v)
     return v;
 }
 
+// One step of a DPP scan of a 64-bit value: both halves moved from the same source lane, then added.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ long long dpp_add64(long long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)v, kCtrl, kRowMask, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)((unsigned long long)v >> 32), kCtrl, kRowMask, 0xF, false);
+    return v + (long long)((unsigned long long)hi << 32 | lo);
+}
+
+// The sum of v over the wave's 64 lanes, on every lane: the inclusive DPP scan above on both halves of the 64-bit
+// value (no LDS round trips), then lane 63's total.
+__device__ __forceinline__ long long wave_total64(long long v)
+{
+    v = dpp_add64<0x111, 0xF>(v);
+    v = dpp_add64<0x112, 0xF>(v);
+    v = dpp_add64<0x114, 0xF>(v);
+    v = dpp_add64<0x118, 0xF>(v);
+    v = dpp_add64<0x142, 0xA>(v);
+    v = dpp_add64<0x143, 0xC>(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)v >> 32), 63);
+    return (long long)((unsigned long long)hi << 32 | lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {

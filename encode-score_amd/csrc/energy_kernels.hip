@@ -449,7 +449,9 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
     const int x = px * 16 + (lane & 3) * 4, y = py * 16 + (lane >> 2);
     const size_t n = (size_t)J.w * J.h, i0 = (size_t)y * J.w + x;
     const int window = (py * 16 / WIN) * ww + px * 16 / WIN;
-    auto plane = [&](int f) { return J.ring + (size_t)(f % J.slots) * n + i0; };
+    // 4 pixels of a plane (4-byte aligned: planes of odd sizes exist)
+    auto quad = [](const int32_t *p) { return make_int4(p[0], p[1], p[2], p[3]); };
+    auto plane = [&](int f) { return quad(J.ring + (size_t)(f % J.slots) * n + i0); };
     int sa[4], sb[4];
     const int c0 = J.centre0;
     if (c0 > kRadius) {
@@ -459,69 +461,71 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
             sb[k] = J.sb[i0 + k];
         }
     }
-    // the previous centre's plane, which is this centre's plane c - 1: read once, kept in registers
-    int m1[4] = {0, 0, 0, 0};
-    if (c0 > kRadius) {
-        const int32_t *p1 = plane(c0 - 1);
+    // The source planes of frames c - 7 .. c + 6 of centre c, in registers: frame f in slot (f - c0 + 7) mod 14.
+    // Each centre uses frames c - 7, c - 1, c and c + 6 and then loads frame c + 7 into the slot of frame c - 7, so
+    // every plane is read once per patch (the row loop over centres is unrolled 14 times, so the slots are fixed).
+    int4 R[14];
 #pragma unroll
-        for (int k = 0; k < 4; k++) m1[k] = p1[k];
-    }
-    for (int ci = 0; ci < J.ncentres; ci++) {
-        const int c = c0 + ci;
-        int m0[4];
-        if (c == kRadius) {  // the first centre of the run: the sums in full
-            const int32_t *p[13];
+    for (int q = 0; q < 14; q++) R[q] = c0 - 7 + q >= 0 ? plane(c0 - 7 + q) : make_int4(0, 0, 0, 0);
+    auto get = [](const int4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; };
+    for (int ci0 = 0; ci0 < J.ncentres; ci0 += 14) {
 #pragma unroll
-            for (int o = 0; o < 13; o++) p[o] = plane(c + o - kRadius);
+        for (int u = 0; u < 14; u++) {
+            const int ci = ci0 + u;
+            if (ci >= J.ncentres) break;
+            const int c = c0 + ci;
+            const int4 &mc = R[(u + 7) % 14], &m6 = R[(u + 13) % 14], &m7 = R[u], &m1 = R[(u + 6) % 14];
+            int m0[4];
+            if (c == kRadius) {  // the first centre of the run: the sums in full, frames 0 .. 12 in slots 1 .. 13
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    sa[k] = get(R[1], k) + get(R[3], k) + get(R[5], k) + get(R[9], k) + get(R[11], k) + get(R[13], k);
+                    sb[k] = get(R[2], k) + get(R[4], k) + get(R[6], k) + get(R[8], k) + get(R[10], k) + get(R[12], k);
+                    m0[k] = get(R[7], k);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    m0[k] = get(mc, k);
+                    const int a = sb[k] - get(mc, k) + get(m6, k);
+                    const int b = sa[k] - get(m7, k) + get(m1, k);
+                    sa[k] = a;
+                    sb[k] = b;
+                }
+            }
+            // frame c + 7, the next centre's c + 6, into the slot of frame c - 7 (past the run: a valid frame, unused)
+            R[u] = plane(c + 7 < c0 + J.ncentres + kRadius ? c + 7 : c);
+            long long v[4] = {0, 0, 0, 0};
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                sa[k] = p[0][k] + p[2][k] + p[4][k] + p[8][k] + p[10][k] + p[12][k];
-                sb[k] = p[1][k] + p[3][k] + p[5][k] + p[7][k] + p[9][k] + p[11][k];
-                m0[k] = p[6][k];
+                const long long st = (long long)sa[k] + sb[k];
+                v[0] += m0[k] * st;
+                v[1] += (long long)sa[k] * sb[k];
+                v[2] += (long long)sa[k] * sa[k];
+                v[3] += (long long)sb[k] * sb[k];
             }
-        } else {
-            const int32_t *pc = plane(c), *p6 = plane(c + 6), *p7 = plane(c - 7);
+            const int slot = J.slot0 + ci;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                m0[k] = pc[k];
-                const int a = sb[k] - pc[k] + p6[k];
-                const int b = sa[k] - p7[k] + m1[k];
-                sa[k] = a;
-                sb[k] = b;
+                const long long t = wave_total64(v[k]);
+                if (lane == 0) {
+                    long long *o = J.psrc + ((size_t)slot * ww * wh + window) * 4 + k;
+                    if (WIN == 16) *o = t;
+                    else atomicAdd((unsigned long long *)o, (unsigned long long)t);
+                }
             }
-        }
+            for (int e = 0; e < J.nenc; e++) {
+                if (c > J.last_centre[e]) continue;
+                const int4 pe = quad(J.emid[e] + (size_t)(c % J.slots) * n + i0);
+                long long w = 0;
 #pragma unroll
-        for (int k = 0; k < 4; k++) m1[k] = m0[k];
-        long long v[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const long long st = (long long)sa[k] + sb[k];
-            v[0] += m0[k] * st;
-            v[1] += (long long)sa[k] * sb[k];
-            v[2] += (long long)sa[k] * sa[k];
-            v[3] += (long long)sb[k] * sb[k];
-        }
-        const int slot = J.slot0 + ci;
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const long long t = reduce_lanes(v[k], 64);
-            if (lane == 0) {
-                long long *o = J.psrc + ((size_t)slot * ww * wh + window) * 4 + k;
-                if (WIN == 16) *o = t;
-                else atomicAdd((unsigned long long *)o, (unsigned long long)t);
-            }
-        }
-        for (int e = 0; e < J.nenc; e++) {
-            if (c > J.last_centre[e]) continue;
-            const int32_t *pe = J.emid[e] + (size_t)(c % J.slots) * n + i0;
-            long long u = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) u += (long long)pe[k] * ((long long)sa[k] + sb[k]);
-            u = reduce_lanes(u, 64);
-            if (lane == 0) {
-                long long *o = J.penc[e] + (size_t)slot * ww * wh + window;
-                if (WIN == 16) *o = u;
-                else atomicAdd((unsigned long long *)o, (unsigned long long)u);
+                for (int k = 0; k < 4; k++) w += (long long)get(pe, k) * ((long long)sa[k] + sb[k]);
+                w = wave_total64(w);
+                if (lane == 0) {
+                    long long *o = J.penc[e] + (size_t)slot * ww * wh + window;
+                    if (WIN == 16) *o = w;
+                    else atomicAdd((unsigned long long *)o, (unsigned long long)w);
+                }
             }
         }
     }
