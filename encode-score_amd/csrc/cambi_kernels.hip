@@ -273,11 +273,17 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
     if (strip >= across * ((h + kScaleRows - 1) / kScaleRows)) return;  // uniform over the wave
     const int x0 = (strip % across) * 64, y0 = (strip / across) * kScaleRows, y1 = min(y0 + kScaleRows, h);
     const int x = x0 + lane;
-    const ES_GLOBAL uint16_t *prev = (const ES_GLOBAL uint16_t *)(pic.planes + pic.off[s - 1]);
-    ES_GLOBAL uint16_t *out = (ES_GLOBAL uint16_t *)(pic.planes + pic.off[s]);
-    // this lane's column and its halo column (lane 0: the column left of the strip, lane 63: the one right of it)
-    const int xc = 2 * min(x, w - 1), xh = 2 * (lane == 0 ? max(x0 - 1, 0) : min(x0 + 64, w - 1));
-    auto row = [&](int r) { return (size_t)(2 * min(max(r, 0), h - 1)) * pw; };
+    // the previous scale's packed plane and this scale's as buffers: rows at a scalar offset, and a lane right of the
+    // picture stores past the end, where the store is dropped
+    const __amdgpu_buffer_rsrc_t in = buffer_over(pic.planes + pic.off[s - 1], (unsigned long long)pw * pic.sh[s - 1] * 2);
+    const __amdgpu_buffer_rsrc_t out = buffer_over(pic.planes + pic.off[s], (unsigned long long)w * h * 2);
+    // this lane's column and its halo column (lane 0: the column left of the strip, lane 63: the one right of it), as
+    // byte offsets of the decimated samples (2x, 2y) of the previous scale
+    const unsigned vc = 4u * (unsigned)min(x, w - 1), vh = 4u * (unsigned)(lane == 0 ? max(x0 - 1, 0) : min(x0 + 64, w - 1));
+    const unsigned vo = x < w ? 2u * (unsigned)x : 0x80000000u;
+    auto ld = [&](unsigned v, int r) {
+        return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(in, v, (unsigned)(2 * min(max(r, 0), h - 1)) * (unsigned)pw * 2u, 0);
+    };
     // the horizontal 3-mode of a row from its own and halo samples; the picture's first and last columns keep theirs
     auto hmode = [&](unsigned c, unsigned hv) -> unsigned {
         const unsigned v = c & 0x7FFFu, e = hv & 0x7FFFu;
@@ -285,23 +291,36 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
         const unsigned l = lane == 0 ? e : left, rt = lane == 63 ? e : right;
         return (x == 0 || x >= w - 1) ? v : mode3(l, v, rt);
     };
-    // rows y0 - 1 and y0 to start: t = horizontal modes, c = raw samples (value and mask bit)
-    unsigned c0 = prev[row(y0 - 1) + xc], e0 = prev[row(y0 - 1) + xh];
-    unsigned c1 = prev[row(y0) + xc], e1 = prev[row(y0) + xh];
-    unsigned t0 = hmode(c0, e0), t1 = hmode(c1, e1);
-    unsigned nc = prev[row(y0 + 1) + xc], ne = prev[row(y0 + 1) + xh];
-    for (int y = y0; y < y1; y++) {
-        // row y + 1 is in; load row y + 2 for the next step
-        const unsigned c2 = nc, e2 = ne;
-        nc = prev[row(y + 2) + xc];
-        ne = prev[row(y + 2) + xh];
+    // rows y - 1 and y: t = horizontal modes, c = raw sample (value and mask bit); rows y + 1 .. y + 3 in flight in
+    // slots (row - y0 - 1) mod 3, each loaded three rows before it is used (the loop is unrolled 3 times)
+    unsigned C[3], E[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        C[k] = ld(vc, y0 + 1 + k);
+        E[k] = ld(vh, y0 + 1 + k);
+    }
+    unsigned c1 = ld(vc, y0);
+    unsigned t0 = hmode(ld(vc, y0 - 1), ld(vh, y0 - 1)), t1 = hmode(c1, ld(vh, y0));
+    auto step = [&](const int y, const int i) __attribute__((always_inline)) {
+        const unsigned c2 = C[i], e2 = E[i];
+        C[i] = ld(vc, y + 4);
+        E[i] = ld(vh, y + 4);
         const unsigned t2 = hmode(c2, e2);
         const unsigned m = (y == 0 || y == h - 1) ? (c1 & 0x7FFFu) : mode3(t0, t1, t2);
-        if (x < w) out[(size_t)y * w + x] = (uint16_t)(m | (c1 & 0x8000u));
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(m | (c1 & 0x8000u)), out, vo,
+                                              __builtin_amdgcn_readfirstlane(2u * (unsigned)y * (unsigned)w), 0);
         t0 = t1;
         t1 = t2;
         c1 = c2;
+    };
+    int y = y0;
+    for (; y + 2 < y1; y += 3) {
+        step(y, 0);
+        step(y + 1, 1);
+        step(y + 2, 2);
     }
+    if (y < y1) step(y, 0);
+    if (y + 1 < y1) step(y + 1, 1);
 }
 
 void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, hipStream_t st)
