@@ -213,29 +213,31 @@ void launch_color_sums(const uint16_t *u, const uint16_t *v, size_t n, int up, u
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// frame_sums: per block kFrameSumRows luma rows and half as many chroma rows of one job.  Each lane reads kSumUnroll
-// columns 256 apart before using any (clamped to the row, the surplus ones counted as zero), so a wave keeps many
-// loads in flight; the sample sizes of the two pictures are template parameters, chosen once per block.
+// frame_sums: per block kFrameSumRows luma rows and half as many chroma rows of one job.  A lane takes kSumUnroll
+// columns 256 apart (clamped to the row, the surplus ones counted as zero) and walks down the block's rows two at a
+// time, so that a wave keeps many loads in flight.  Planes are read as buffers: a column's byte offset is fixed for
+// the walk and a row's offset is one scalar.  The sample sizes of the two pictures are template parameters, chosen
+// once per block.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int kSumUnroll = 4;
 
+// A plane of BYTES bytes per sample as a buffer over the `rows` rows the block may read: sample (column offset vo,
+// row y) = (word >> rshift) << shift.
 template <int BYTES>
-__device__ __forceinline__ unsigned luma_at(const PlaneIn &in, int x, int y)
-{
-    const size_t i = (size_t)y * in.pitch + x;
-    const unsigned v = BYTES == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)in.ptr)[i]
-                                  : (unsigned)((const ES_GLOBAL uint16_t *)in.ptr)[i] >> in.rshift;
-    return v << in.shift;
-}
-
-template <int BYTES>
-__device__ __forceinline__ unsigned chroma_at(const ChromaIn &c, int x, int y)
-{
-    const size_t i = (size_t)y * c.pitch + (size_t)x * c.step;
-    const unsigned v = BYTES == 1 ? (unsigned)((const ES_GLOBAL uint8_t *)c.ptr)[i]
-                                  : (unsigned)((const ES_GLOBAL uint16_t *)c.ptr)[i] >> c.rshift;
-    return v << c.shift;
-}
+struct SumPlane {
+    __amdgpu_buffer_rsrc_t rs;
+    unsigned pitch;  // bytes per row
+    int rshift, shift;
+    __device__ __forceinline__ SumPlane(const void *p, int pitch_samples, unsigned long long samples, int rsh, int sh)
+        : rs(buffer_over(p, samples * BYTES)), pitch((unsigned)pitch_samples * BYTES), rshift(rsh), shift(sh) {}
+    __device__ __forceinline__ unsigned at(unsigned vo, int y) const
+    {
+        const unsigned so = __builtin_amdgcn_readfirstlane((unsigned)y * pitch);
+        const unsigned v = BYTES == 1 ? (unsigned)__builtin_amdgcn_raw_buffer_load_b8(rs, vo, so, 0)
+                                      : (unsigned)__builtin_amdgcn_raw_buffer_load_b16(rs, vo, so, 0);
+        return (v >> rshift) << shift;
+    }
+};
 
 template <int BR, int BD>
 __device__ __forceinline__ void frame_sums_block(const FrameSumJob &J, int r0)
@@ -243,40 +245,79 @@ __device__ __forceinline__ void frame_sums_block(const FrameSumJob &J, int r0)
     constexpr int U = kSumUnroll;
     const int w = J.w, r1 = min(r0 + kFrameSumRows, J.h);
     unsigned long long s[kFrameSums] = {};
-    for (int y = r0; y < r1; y++)
+    {
+        // the planes' samples up to row r1 - 1, column w - 1
+        const unsigned long long nr = (unsigned long long)(r1 - 1) * J.ry.pitch + w, nd = (unsigned long long)(r1 - 1) * J.dy.pitch + w;
+        const SumPlane<BR> R(J.ry.ptr, J.ry.pitch, nr, J.ry.rshift, J.ry.shift);
+        const SumPlane<BD> D(J.dy.ptr, J.dy.pitch, nd, J.dy.rshift, J.dy.shift);
         for (int xb = threadIdx.x; xb < w; xb += 256 * U) {
-            unsigned a[U], b[U];
+            unsigned cr[U], cd[U];
+            bool in[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const int x = min(xb + 256 * k, w - 1);
-                a[k] = luma_at<BR>(J.ry, x, y);
-                b[k] = luma_at<BD>(J.dy, x, y);
+                const int x = xb + 256 * k;
+                in[k] = x < w;
+                cr[k] = (unsigned)min(x, w - 1) * BR;
+                cd[k] = (unsigned)min(x, w - 1) * BD;
             }
+            for (int y = r0; y < r1; y += 2) {
+                const int y2 = min(y + 1, r1 - 1);  // a lone last row is read twice and counted once
+                unsigned a[2][U], b[2][U];
 #pragma unroll
-            for (int k = 0; k < U; k++) s[0] += xb + 256 * k < w ? sq_diff(a[k], b[k]) : 0u;
+                for (int k = 0; k < U; k++) {
+                    a[0][k] = R.at(cr[k], y);
+                    b[0][k] = D.at(cd[k], y);
+                    a[1][k] = R.at(cr[k], y2);
+                    b[1][k] = D.at(cd[k], y2);
+                }
+#pragma unroll
+                for (int k = 0; k < U; k++) {
+                    s[0] += in[k] ? sq_diff(a[0][k], b[0][k]) : 0u;
+                    s[0] += in[k] && y + 1 < r1 ? sq_diff(a[1][k], b[1][k]) : 0u;
+                }
+            }
         }
+    }
     const int w2 = w >> 1, c0 = r0 >> 1, c1 = r1 >> 1, up = J.up;
-    const bool src_color = J.src_color;
-    for (int y = c0; y < c1; y++)
+    if (c1 > c0) {
+        const bool src_color = J.src_color;
+        auto plane = [&](const ChromaIn &c, auto bytes) {
+            using P = SumPlane<decltype(bytes)::value>;
+            return P(c.ptr, c.pitch, (unsigned long long)(c1 - 1) * c.pitch + (unsigned long long)(w2 - 1) * c.step + 1, c.rshift,
+                     c.shift);
+        };
+        const auto RU = plane(J.ru, std::integral_constant<int, BR>()), RV = plane(J.rv, std::integral_constant<int, BR>());
+        const auto DU = plane(J.du, std::integral_constant<int, BD>()), DV = plane(J.dv, std::integral_constant<int, BD>());
         for (int xb = threadIdx.x; xb < w2; xb += 256 * U) {
-            unsigned ur[U], vr[U], ud[U], vd[U];
+            unsigned crr[U], cdd[U];
+            bool in[U];
 #pragma unroll
             for (int k = 0; k < U; k++) {
-                const int x = min(xb + 256 * k, w2 - 1);
-                ur[k] = chroma_at<BR>(J.ru, x, y);
-                vr[k] = chroma_at<BR>(J.rv, x, y);
-                ud[k] = chroma_at<BD>(J.du, x, y);
-                vd[k] = chroma_at<BD>(J.dv, x, y);
+                const int x = xb + 256 * k;
+                in[k] = x < w2;
+                crr[k] = (unsigned)min(x, w2 - 1) * J.ru.step * BR;  // u and v of a picture share their step
+                cdd[k] = (unsigned)min(x, w2 - 1) * J.du.step * BD;
             }
+            for (int y = c0; y < c1; y++) {
+                unsigned ur[U], vr[U], ud[U], vd[U];
 #pragma unroll
-            for (int k = 0; k < U; k++) {
-                if (xb + 256 * k >= w2) break;  // the lane's columns are in order: the rest are past the row too
-                s[1] += sq_diff(ur[k], ud[k]);
-                s[2] += sq_diff(vr[k], vd[k]);
-                color_add(ud[k] << up, vd[k] << up, s[3], s[4], s[5]);
-                if (src_color) color_add(ur[k] << up, vr[k] << up, s[6], s[7], s[8]);
+                for (int k = 0; k < U; k++) {
+                    ur[k] = RU.at(crr[k], y);
+                    vr[k] = RV.at(crr[k], y);
+                    ud[k] = DU.at(cdd[k], y);
+                    vd[k] = DV.at(cdd[k], y);
+                }
+#pragma unroll
+                for (int k = 0; k < U; k++) {
+                    if (!in[k]) break;  // the lane's columns are in order: the rest are past the row too
+                    s[1] += sq_diff(ur[k], ud[k]);
+                    s[2] += sq_diff(vr[k], vd[k]);
+                    color_add(ud[k] << up, vd[k] << up, s[3], s[4], s[5]);
+                    if (src_color) color_add(ur[k] << up, vr[k] << up, s[6], s[7], s[8]);
+                }
             }
         }
+    }
     block_add<kFrameSums>(s, J.out);
 }
 
