@@ -219,7 +219,7 @@ void launch_color_sums(const uint16_t *u, const uint16_t *v, size_t n, int up, u
 // the walk and a row's offset is one scalar.  The sample sizes of the two pictures are template parameters, chosen
 // once per block.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int kSumUnroll = 4;
+constexpr int kSumUnroll = 8;
 
 // A plane of BYTES bytes per sample as a buffer over the `rows` rows the block may read: sample (column offset vo,
 // row y) = (word >> rshift) << shift.
