@@ -771,6 +771,11 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
                 float c = 0.0f;
 #pragma unroll
                 for (int d = 0; d < kDiffs; d++) {
+                    // the thresholds (value <= tvi[d], value + d + 1 > vlt) combined without short-circuiting; a
+                    // difference no served pixel of the row passes is skipped by the whole wave (bright rows pass
+                    // only the larger differences)
+                    const bool ok = (value <= t.tvi[d]) & (value + d + 1 > t.vlt);
+                    if (!__builtin_amdgcn_ballot_w64(serve & ok)) continue;
                     const unsigned pa = cnt9[kDiffs + d + 1], pb = cnt9[kDiffs - d - 1];
                     const unsigned pm = pa > pb ? pa : pb;
                     // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; computed on
@@ -780,8 +785,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
                     const float rcp = recip_rn(p0 + pm);
                     float val = __fmul_rn((float)__umul24(__umul24((unsigned)t.weight[d], p0), pm), rcp);
                     asm("" : "+v"(val));
-                    // the tests combined without short-circuiting, so that they are compares and one select
-                    const bool take = (value <= t.tvi[d]) & (value + d + 1 > t.vlt) & (val > c);
+                    const bool take = ok & (val > c);
                     c = take ? val : c;
                 }
                 c = serve ? c : 0.0f;
