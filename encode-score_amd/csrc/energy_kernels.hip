@@ -229,13 +229,15 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
             pr[k][p] = sample(p, ca[p], y0 - 4 + k);
             ph[k][p] = sample(p, ch[p], y0 - 4 + k);
         }
-    int h3c[2][9], h3l[2][3], h3r[2][3], h9[2][9], yv[2][3], s9[2];
+    // per picture: horizontal 3-sums centred on the column (h3c) and the horizontal 9-sums (h9) of the last 9 rows,
+    // the differences of the 3-sums centred right and left of it (h3d) and the samples (yv) of the last 3 rows
+    int h3c[2][9], h3d[2][3], h9[2][9], yv[2][3], s9[2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
 #pragma unroll
         for (int k = 0; k < 9; k++) h3c[p][k] = h9[p][k] = 0;
 #pragma unroll
-        for (int k = 0; k < 3; k++) h3l[p][k] = h3r[p][k] = yv[p][k] = 0;
+        for (int k = 0; k < 3; k++) h3d[p][k] = yv[p][k] = 0;
         s9[p] = 0;
     }
     int accF[3] = {0, 0, 0}, accG[6] = {0, 0, 0, 0, 0, 0};
@@ -294,9 +296,9 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
                 int v[9];
 #pragma unroll
                 for (int d = 0; d < 9; d++) v[d] = rowbuf[buf][p][lane + d];  // columns x - 4 .. x + 4
-                const int a = v[3] + v[4];
-                const int hc = a + v[5], hl = a + v[2], hr = v[4] + v[5] + v[6];
-                const int h9v = hl + (v[5] + v[6]) + (v[0] + v[1]) + (v[7] + v[8]);
+                const int a = v[3] + v[4], b56 = v[5] + v[6];
+                const int hc = a + v[5], hd = b56 - (v[2] + v[3]);  // (v4 + v5 + v6) - (v2 + v3 + v4)
+                const int h9v = (a + v[2]) + b56 + (v[0] + v[1]) + (v[7] + v[8]);
                 // slots: row r - k sits in h3c / h9 slot (i - k) mod 9 and in the 3-slot rings at (i - k) mod 3
                 const int c1 = h3c[p][(i + 8) % 9], c2 = h3c[p][(i + 7) % 9], c3 = h3c[p][(i + 6) % 9];
                 const int c4 = h3c[p][(i + 5) % 9], c5 = h3c[p][(i + 4) % 9];
@@ -304,15 +306,14 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
                 F[p] = 9 * yv[p][(i + 2) % 3] - (c2 + c1 + hc);
                 // row r - 2: GX = S3(x + 1) - S3(x - 1), GY = S3(y + 1) - S3(y - 1), zero on the picture's border
                 const int yg = r - 2;
-                GX[p] = (x > 0 && x < w - 1) ? (h3r[p][0] + h3r[p][1] + h3r[p][2]) - (h3l[p][0] + h3l[p][1] + h3l[p][2]) : 0;
+                GX[p] = (x > 0 && x < w - 1) ? h3d[p][0] + h3d[p][1] + h3d[p][2] : 0;
                 GY[p] = (yg > 0 && yg < h - 1) ? (hc + c1) - (c3 + c4) : 0;
                 // row r - 4: M = 9 S3 - S9
                 s9[p] += h9v - h9[p][i];
                 M[p] = 9 * (c5 + c4 + c3) - s9[p];
                 h3c[p][i] = hc;
                 h9[p][i] = h9v;
-                h3l[p][i % 3] = hl;
-                h3r[p][i % 3] = hr;
+                h3d[p][i % 3] = hd;
                 yv[p][i % 3] = v[4];
             }
             buf ^= 1;
