@@ -50,9 +50,11 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     }
     jobs_.alloc(sizeof jobs);
     hip_check(hipMemcpy(jobs_.p, jobs, sizeof jobs, hipMemcpyHostToDevice), "top-k jobs");
+    // the c-value tiles, coarsest scale first (slot k holds scale kScales - 1 - k)
     int first[kScales + 1];
     first[0] = 0;
-    for (int s = 0; s < kScales; s++) first[s + 1] = first[s] + cvalue_tiles(info_.sw[s], info_.sh[s], tile_h_);
+    for (int k = 0; k < kScales; k++)
+        first[k + 1] = first[k] + cvalue_tiles(info_.sw[kScales - 1 - k], info_.sh[kScales - 1 - k], tile_h_);
     tiles_ = first[kScales];
     base_tiles_ = base_tiles(w, h);
     first_.alloc(sizeof first);

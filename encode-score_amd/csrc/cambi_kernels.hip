@@ -1000,8 +1000,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCvWaves<Sh>
             t = __builtin_amdgcn_readfirstlane((int)v);
         }
         if (t >= total) break;
-        const int slot = find_slot(tile_first, nslots, t);
-        const int p = slot / kScales, s = slot % kScales;
+        const int slot = find_slot(tile_first, nslots, t);  // slots by scale, coarsest first, then by picture
+        const int np = nslots / kScales, p = slot % np, s = kScales - 1 - slot / np;
         cv_tile(sh, pics[p], s, t - tile_first[slot], tile_h);
         if (!persistent) break;
     }

@@ -98,7 +98,9 @@ inline long long cambi_list_layout(const int *sw, const int *sh, int tile_h, lon
 void launch_cambi_base(const CambiPic *pics, const int *tile_first, int npics, int total_tiles, hipStream_t st);
 int scale_strips(int w, int h);
 void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, hipStream_t st);
-// tile_first[nslots] (device) holds the total; total_tiles sizes the grid when not persistent.
+// tile_first[k] (device) is the first tile of slot k, which holds scale kScales - 1 - k / npics of picture k % npics
+// (nslots = npics * kScales: coarsest scale first); tile_first[nslots] holds the total; total_tiles sizes the grid
+// when not persistent.
 // mfma: window counts on the matrix cores instead of prefix scans (same results).
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
                           int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma = 0);

@@ -751,8 +751,6 @@ private:
                 const CambiInfo &ci = crops[c].ci;
                 const bool on = i < n && (r < C ? crop_active(r, f) && !pend[p].hit[(size_t)i * C + r] : active(r - C, f));
                 for (int s = 0; s < kScales; s++) {
-                    first[pidx * kScales + s] = t;
-                    if (on) t += cvalue_tiles(ci.sw[s], ci.sh[s], tile_h);
                     pc.sw[s] = on ? ci.sw[s] : 0;
                     pc.sh[s] = on ? ci.sh[s] : 0;
                     pc.off[s] = ci.off[s];
@@ -775,6 +773,14 @@ private:
                 pc.consts = consts.as<CambiConsts>() + c;
             }
         }
+        // the c-value tiles by scale, coarsest first, then by picture: a coarse tile holds many more relevant values
+        // (passes) than a fine one, so the long tiles are handed out first and the launch ends on short ones
+        for (int k = 0; k < kScales; k++)
+            for (int pidx = 0; pidx < P; pidx++) {
+                const int s = kScales - 1 - k;
+                first[k * P + pidx] = t;
+                t += cvalue_tiles(pics[pidx].sw[s], pics[pidx].sh[s], tile_h);
+            }
         first[P * kScales] = t;
         for (int i = 0; i < B; i++) {
             const long f = f0 + i;
