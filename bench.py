@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=48)
     ap.add_argument("--cpu-frames", type=int, default=-1, help="frames of the CPU baseline sample (-1: per config)")
     ap.add_argument("--pool", type=int, default=512, help="distinct frames generated per file at most")
+    ap.add_argument("--solo-rounds", type=int, default=2,
+                    help="rounds of the serial run that times each kernel alone (0: none; the roofline kernel is then "
+                         "the CAMBI chain's longest)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", type=int, default=1, help="run the autotuner before the timed region (1) or not (0)")
@@ -289,8 +292,34 @@ def main():
         "frame_sums": 1.5 * (sbytes * pix["src"] + 2 * pix["enc"]) * B if args.chroma else 0,
     }
     launches = max(rounds, 1)
-    dominant = max(alg_bytes, key=lambda k: stage[k])
-    mean_ms = stage[dominant] / launches
+    # Which kernel dominates, from each kernel's time alone on the GPU.  With the chains overlapped, a stage of the
+    # energy chain (on the side stream) waits there for the CUs the c-value kernel's persistent waves hold, so its
+    # event span is not its kernel time: a short serial run of the same scorer configuration (one untimed round,
+    # then --solo-rounds, stage events around each kernel) gives the solo times.  The roofline then takes the
+    # dominant kernel's mean launch time from the timed region when it runs on the main stream there (the CAMBI
+    # chain, or every stage when serial), and from the serial run otherwise.
+    if args.serial:
+        solo, solo_launches = stage, launches
+    elif args.solo_rounds <= 0:  # no serial run: the main stream's stages only
+        solo = {k: (stage[k] if k.startswith("cambi_") else 0.0) for k in stage}
+        solo_launches = launches
+    else:
+        cal = es.Scorer(es.ES_BATCHED, W, H, descs, batch=batch, tile_h=tile_h, use_graph=True, profile=True,
+                        src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves, serial_stages=True,
+                        cv_tile_blocks=cv_tb, early_energies=early, cv_mfma=mfma)
+        submit(cal, 0, batch)
+        cal.sync()
+        s0, r0 = cal.stage_ms()
+        submit(cal, batch, args.solo_rounds * batch)
+        cal.sync()
+        s1, r1 = cal.stage_ms()
+        solo = {k: s1[k] - s0[k] for k in s1}
+        solo_launches = max(r1 - r0, 1)
+        del cal
+    dominant = max(alg_bytes, key=lambda k: solo[k])
+    on_main = args.serial or dominant.startswith("cambi_")
+    mean_ms = stage[dominant] / launches if on_main else solo[dominant] / solo_launches
+    solo_ms = {k: round(solo[k] / solo_launches, 5) for k in alg_bytes}
     achieved = alg_bytes[dominant] / (mean_ms / 1e3) / 1e9
     traffic, traffic_src = measured_traffic(args.config, W, H, batch, dominant, cv_bmax or 32, mfma)
     # the whole step's input bytes (each luma plane, and with chroma each chroma plane, read once) against HBM
@@ -325,7 +354,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "mean_launch_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes[dominant])},
+                     "mean_launch_ms": round(mean_ms, 5), "algorithmic_bytes_per_launch": int(alg_bytes[dominant]),
+                     "timing": "timed region" if on_main else "serial run (the kernel alone)",
+                     "solo_launch_ms": solo_ms},
     }
 
     # the source-CAMBI cache (src/cache.cpp): a first run against the source fills it, a second run reads every
