@@ -506,11 +506,10 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
                 v[3] += (long long)sb[k] * sb[k];
             }
             const int slot = J.slot0 + ci;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const long long t = wave_total64(v[k]);
-                if (lane == 0) {
-                    long long *o = J.psrc + ((size_t)slot * ww * wh + window) * 4 + k;
+            {
+                const long long t = wave_total4x64(v);  // value k in lane 16 k + 15
+                if ((lane & 15) == 15) {
+                    long long *o = J.psrc + ((size_t)slot * ww * wh + window) * 4 + (lane >> 4);
                     if (WIN == 16) *o = t;
                     else atomicAdd((unsigned long long *)o, (unsigned long long)t);
                 }
