@@ -1065,40 +1065,49 @@ __global__ __launch_bounds__(256) void topk_count_kernel(TopkJob *jobs, int pass
     }
 }
 
-// One block per job: the digit counts and sums of the counting blocks added up, then the walk from the top digit.
+// One block per job: the digit counts and sums of the counting blocks added up, then the walk from the top digit,
+// in parallel.  Thread t holds digit g = 511 - t, so an inclusive scan over the threads (DPP within each wave, the
+// waves' totals through LDS) gives each digit the count of keys at or above it (incl) and strictly above it (excl).
+// The host loop (src/cambi.cpp:151-156) stops at the highest digit g >= 1 with excl + count >= remaining, or at 0:
+// since incl only grows with t, that is the one thread with (t == 511 or incl >= remaining) and (t == 0 or
+// excl < remaining), which records the digit and the count and exact sum of the keys above it.
 __global__ __launch_bounds__(512) void topk_walk_kernel(TopkJob *jobs, int pass, int blocks)
 {
     TopkJob &J = jobs[blockIdx.x];
     if ((unsigned long long)*J.count <= J.k) return;
-    __shared__ unsigned hist[512];
-    __shared__ unsigned long long hs[512];
-    const int d = threadIdx.x;
+    __shared__ unsigned wc[8];
+    __shared__ unsigned long long wf[8];
+    const int t = threadIdx.x, g = 511 - t, lane = t & 63, wave = t >> 6;
+    // the job's state, read by every thread before the barrier below, after which one thread rewrites it
+    const unsigned remaining = pass ? J.remaining : (unsigned)J.k;
+    const unsigned prefix = pass ? J.prefix : 0u;
+    const unsigned long long above0 = pass ? J.above : 0ull, fx0 = pass ? J.above_fx : 0ull;
     unsigned c = 0;
     unsigned long long sm = 0;
 #pragma unroll 4
     for (int b = 0; b < blocks; b++) {
-        c += J.hist[(size_t)b * 512 + d];
-        sm += J.hsum[(size_t)b * 512 + d];
+        c += J.hist[(size_t)b * 512 + g];
+        sm += J.hsum[(size_t)b * 512 + g];
     }
-    hist[d] = c;
-    hs[d] = sm;
+    const unsigned ci = wave_scan_incl(c);
+    const unsigned long long fi = (unsigned long long)wave_scan_incl64((long long)sm);
+    if (lane == 63) {
+        wc[wave] = ci;
+        wf[wave] = fi;
+    }
     __syncthreads();
-    if (d == 0) {
-        const int shift = 18 - 9 * pass;
-        const unsigned prefix = pass ? J.prefix : 0u;
-        const unsigned remaining = pass ? J.remaining : (unsigned)J.k;
-        unsigned above = 0;
-        unsigned long long above_fx = 0;
-        int g = 511;
-        for (; g > 0; g--) {
-            if (above + hist[g] >= remaining) break;
-            above += hist[g];
-            above_fx += hs[g];
-        }
-        J.remaining = remaining - above;
-        J.prefix = prefix | ((unsigned)g << shift);
-        J.above = (pass ? J.above : 0ull) + above;
-        J.above_fx = (pass ? J.above_fx : 0ull) + above_fx;
+    unsigned off = 0;
+    unsigned long long foff = 0;
+    for (int v = 0; v < wave; v++) {
+        off += wc[v];
+        foff += wf[v];
+    }
+    const unsigned incl = off + ci, excl = incl - c;
+    if ((t == 511 || incl >= remaining) && (t == 0 || excl < remaining)) {
+        J.remaining = remaining - excl;
+        J.prefix = prefix | ((unsigned)g << (18 - 9 * pass));
+        J.above = above0 + excl;
+        J.above_fx = fx0 + (foff + fi - sm);
     }
 }
 

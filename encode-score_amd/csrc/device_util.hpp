@@ -67,9 +67,8 @@ __device__ __forceinline__ long long dpp_add64(long long v)
     return v + (long long)((unsigned long long)hi << 32 | lo);
 }
 
-// The sum of v over the wave's 64 lanes, on every lane: the inclusive DPP scan above on both halves of the 64-bit
-// value (no LDS round trips), then lane 63's total.
-__device__ __forceinline__ long long wave_total64(long long v)
+// Inclusive prefix sum of a 64-bit value over the wave's lanes: the DPP steps of wave_scan_incl on both halves.
+__device__ __forceinline__ long long wave_scan_incl64(long long v)
 {
     v = dpp_add64<0x111, 0xF>(v);
     v = dpp_add64<0x112, 0xF>(v);
@@ -77,6 +76,14 @@ __device__ __forceinline__ long long wave_total64(long long v)
     v = dpp_add64<0x118, 0xF>(v);
     v = dpp_add64<0x142, 0xA>(v);
     v = dpp_add64<0x143, 0xC>(v);
+    return v;
+}
+
+// The sum of v over the wave's 64 lanes, on every lane: the inclusive DPP scan above on both halves of the 64-bit
+// value (no LDS round trips), then lane 63's total.
+__device__ __forceinline__ long long wave_total64(long long v)
+{
+    v = wave_scan_incl64(v);
     const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)v >> 32), 63);
     return (long long)((unsigned long long)hi << 32 | lo);
