@@ -579,19 +579,20 @@ public:
         desc_bytes = align(P * sizeof(CambiPic)) + align(((size_t)P * kScales + 1) * 4) + align((size_t)B * E * sizeof(WsPair)) +
                      align((size_t)C * sizeof(LastingJob)) + align((size_t)B * E * sizeof(CollectJob)) +
                      align((size_t)B * E * sizeof(FrameSumJob));
-        desc.alloc(desc_bytes);
+        for (auto &d : desc) d.alloc(desc_bytes);
         res_bytes = align((size_t)P * 8) + align((size_t)B * E * 12 * 8) + align((size_t)B * E * 4 * 8) +
                     align((size_t)B * E * 3 * 8) + align((size_t)B * E * 8) + align((size_t)B * E * kFrameSums * 8);
         for (int k = 0; k < 2; k++) {
             stage_buf[k].alloc(desc_bytes);
             res[k].alloc(res_bytes);
             hip_check(hipEventCreateWithFlags(&done[k], hipEventDisableTiming), "creating an event");
+            hip_check(hipEventCreateWithFlags(&done_e[k], hipEventDisableTiming), "creating an event");
+            hip_check(hipEventCreateWithFlags(&fork_ev[k], hipEventDisableTiming), "creating an event");
         }
         hip_check(hipStreamCreateWithFlags(&work, hipStreamNonBlocking), "creating a stream");
         hip_check(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "creating a stream");
         hip_check(hipEventCreateWithFlags(&joined, hipEventDisableTiming), "creating an event");
-        hip_check(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming), "creating an event");
-        hip_check(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming), "creating an event");
+        hip_check(hipEventCreateWithFlags(&side_joined, hipEventDisableTiming), "creating an event");
         use_graph = c.use_graph != 0;
         profile = c.profile != 0;
         serial = c.serial_stages != 0;
@@ -605,9 +606,14 @@ public:
     ~BatchedScorer() override
     {
         for (int k = 0; k < 2; k++) {
-            if (done[k]) {
-                (void)hipEventSynchronize(done[k]);
-                (void)hipEventDestroy(done[k]);
+            for (hipEvent_t e : {done[k], done_e[k], fork_ev[k]})
+                if (e) {
+                    (void)hipEventSynchronize(e);
+                    (void)hipEventDestroy(e);
+                }
+            for (int j = 0; j < 3; j++) {
+                if (pexec[k][j]) (void)hipGraphExecDestroy(pexec[k][j]);
+                if (pgraph[k][j]) (void)hipGraphDestroy(pgraph[k][j]);
             }
             if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
             if (graph[k]) (void)hipGraphDestroy(graph[k]);
@@ -621,7 +627,7 @@ public:
                 (void)hipStreamSynchronize(*q);
                 (void)hipStreamDestroy(*q);
             }
-        for (hipEvent_t e : {joined, fork_ev, join_ev})
+        for (hipEvent_t e : {joined, side_joined})
             if (e) (void)hipEventDestroy(e);
         for (auto &row : ev)
             for (auto &pair : row)
@@ -644,7 +650,9 @@ public:
             round(m, src + i, encp + (size_t)i * enc.size(), pts ? pts + i : nullptr, work);
         }
         hip_check(hipEventRecord(joined, work), "event");
+        hip_check(hipEventRecord(side_joined, side), "event");
         hip_check(hipStreamWaitEvent(st, joined, 0), "event");
+        hip_check(hipStreamWaitEvent(st, side_joined, 0), "event");
     }
 
     void sync() override
@@ -696,25 +704,31 @@ private:
     DevBuf consts, planes, list, segs, sums, counts, thist, thsum, topk, scores;
     std::vector<CropDev> cd;
     std::vector<EncDev> ed;
-    DevBuf tot_int, tot_or, tot_last, quant, fsums, scratch, desc, lout;
+    DevBuf tot_int, tot_or, tot_last, quant, fsums, scratch, lout;
+    DevBuf desc[2];  // a round's descriptors, one buffer per round parity: the energy chain of a round can still
+                     // be reading its own while the next round's are uploaded
     DevBuf cwork_int, cwork_dbl, cwork_last, cwork_bins, cwork_fhist;
     CollectWork cwork{};
     size_t lout_words = 0;
     size_t desc_bytes, res_bytes;
     HostBuf stage_buf[2], res[2];
-    hipEvent_t done[2] = {}, joined = nullptr, fork_ev = nullptr, join_ev = nullptr;
+    // done: a round's CAMBI chain and its scores copy (on the work stream); done_e: its energy chain and copies (on the
+    // side stream, unless serial); fork_ev: the point of the work stream where the round's energy chain may start
+    hipEvent_t done[2] = {}, done_e[2] = {}, fork_ev[2] = {}, joined = nullptr, side_joined = nullptr;
     hipEvent_t ev[2][ES_STAGES][2] = {};  // profiled: each stage's start and end on the stream it runs on
     bool profile = false;
     hipStream_t work = nullptr, side = nullptr;  // the round's main stream and the energy chain's
     bool serial = false, early = false;
     hipGraph_t graph[2] = {}, sgraph[2][ES_STAGES] = {};
     hipGraphExec_t exec[2] = {}, sexec[2][ES_STAGES] = {};
+    hipGraph_t pgraph[2][3] = {};  // graph form, chains overlapped: the work stream's two parts and the side's part
+    hipGraphExec_t pexec[2][3] = {};
     Pending pend[2];
     long rounds = 0;
     bool use_graph = false;
 
     template <class T>
-    T *dptr(size_t off) const { return (T *)(desc.as<char>() + off); }
+    T *dptr(int p, size_t off) const { return (T *)(desc[p].as<char>() + off); }
 
     // offsets of the descriptor arrays
     size_t o_pics() const { return 0; }
@@ -863,13 +877,11 @@ private:
     void stage(int k, int p, hipStream_t st)
     {
         const int C = (int)crops.size(), E = (int)enc.size();
-        const CambiPic *pics = dptr<CambiPic>(o_pics());
+        const CambiPic *pics = dptr<CambiPic>(p, o_pics());
         switch (k) {
         case 0:
-            hip_check(hipMemcpyAsync(desc.p, stage_buf[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
+            hip_check(hipMemcpyAsync(desc[p].p, stage_buf[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
             launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
-            if (cfg.win > 16) launch_clear(lout.as<unsigned long long>(), (long long)lout_words, nullptr, 0, st);
-            if (chroma == 1) launch_clear(fsums.as<unsigned long long>(), (long long)B * E * kFrameSums, nullptr, 0, st);
             break;
         case 1: launch_cambi_base(pics, nullptr, P, max_base_tiles, st); break;
         case 2:
@@ -878,24 +890,32 @@ private:
         case 3:
             // persistent waves taking tiles from a counter, or one block per tile of the round's largest layout (the
             // blocks past the round's tiles exit at once)
-            launch_cambi_cvalues(pics, dptr<int>(o_first()), P * kScales, (int)seg_off[P], tile_h, cv_tile_blocks ? 0 : 1,
+            launch_cambi_cvalues(pics, dptr<int>(p, o_first()), P * kScales, (int)seg_off[P], tile_h, cv_tile_blocks ? 0 : 1,
                                  counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st, cv_mfma ? 1 : 0);
             break;
         case 4:
             for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, kTopkBlocks, st);
             break;
-        case 5: launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st); break;
-        case 6: launch_window_stats(dptr<WsPair>(o_ws()), B * E, max_w, max_h, cfg.win, false, st); break;
-        case 7:
-            if (chroma == 1) launch_frame_sums(dptr<FrameSumJob>(o_fs()), B * E, max_h, st);
+        case 5:
+            launch_cambi_finish(pics, topk.as<TopkJob>(), P, scores.as<double>(), st);
+            hip_check(hipMemcpyAsync(res[p].as<char>() + r_scores(), scores.p, (size_t)P * 8, hipMemcpyDeviceToHost, st),
+                      "results");
             break;
-        case 8: launch_lasting(dptr<LastingJob>(o_last()), C, max_w, max_h, cfg.win, st); break;
+        case 6:
+            // the energy chain's own clears (on its stream: the previous round's energy copies are done with them)
+            if (cfg.win > 16) launch_clear(lout.as<unsigned long long>(), (long long)lout_words, nullptr, 0, st);
+            if (chroma == 1) launch_clear(fsums.as<unsigned long long>(), (long long)B * E * kFrameSums, nullptr, 0, st);
+            launch_window_stats(dptr<WsPair>(p, o_ws()), B * E, max_w, max_h, cfg.win, false, st);
+            break;
+        case 7:
+            if (chroma == 1) launch_frame_sums(dptr<FrameSumJob>(p, o_fs()), B * E, max_h, st);
+            break;
+        case 8: launch_lasting(dptr<LastingJob>(p, o_last()), C, max_w, max_h, cfg.win, st); break;
         case 9:
-            launch_collect(dptr<CollectJob>(o_col()), B, E, cwork, st);
+            launch_collect(dptr<CollectJob>(p, o_col()), B, E, cwork, st);
             break;
         case 10: {
             char *r = res[p].as<char>();
-            hip_check(hipMemcpyAsync(r + r_scores(), scores.p, (size_t)P * 8, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_ti(), tot_int.p, tot_int.bytes, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_to(), tot_or.p, tot_or.bytes, hipMemcpyDeviceToHost, st), "results");
             hip_check(hipMemcpyAsync(r + r_tl(), tot_last.p, tot_last.bytes, hipMemcpyDeviceToHost, st), "results");
@@ -916,33 +936,33 @@ private:
         return e;
     }
 
-    // The order of the round's stages: the descriptor upload first; then the CAMBI chain (stages 1-5) on `st` and the
-    // energy chain (6-9: window stats, frame sums, lasting texture, collect) on the side stream, which need nothing
-    // from each other: the energies overlap the c-value kernel (VALU-bound while they are HBM-bound); the results
-    // copy after both.  With serial set, all on `st` in order.
+    // The order of the round's stages.  Serial: all on `st` in order.  Otherwise the CAMBI chain (stages 0-5, the
+    // scores copy with the finish) runs on `st` and the energy chain (6-10: its clears, window stats, frame sums,
+    // lasting texture, collect and its copies) on the side stream; they need nothing from each other.  The energy
+    // chain is released at a point of the CAMBI chain (after the pyramid, or after the upload when early) and only
+    // the host and the caller wait for it, so the next round's CAMBI chain (upload, pyramid) runs beside this round's
+    // energies instead of after them.  The buffers the two chains write are their own (the descriptors one set per
+    // round parity, which the host reuses only after both chains of that round are done).  part(i, k0, k1, stream)
+    // runs stages k0 .. k1 - 1 as part i.
     template <class F>
-    void schedule(hipStream_t st, F &&run)
+    void schedule(int p, hipStream_t st, F &&part)
     {
-        run(0, st);
         if (serial) {
-            for (int k = 1; k < ES_STAGES; k++) run(k, st);
+            part(0, 0, ES_STAGES, st);
+            hip_check(hipEventRecord(done_e[p], st), "event");
             return;
         }
-        // the energy chain starts once the CAMBI pyramid is built (base and scale are HBM-bound like it, the c-value
-        // kernel is not), or with it when the energies are long next to the c-values (early)
         const int fork_after = early ? 0 : 2;
-        for (int k = 1; k <= fork_after; k++) run(k, st);
-        hip_check(hipEventRecord(fork_ev, st), "event");
-        hip_check(hipStreamWaitEvent(side, fork_ev, 0), "event");
-        for (int k = fork_after + 1; k <= 5; k++) run(k, st);
-        for (int k = 6; k <= 9; k++) run(k, side);
-        hip_check(hipEventRecord(join_ev, side), "event");
-        hip_check(hipStreamWaitEvent(st, join_ev, 0), "event");
-        run(10, st);
+        part(0, 0, fork_after + 1, st);
+        hip_check(hipEventRecord(fork_ev[p], st), "event");
+        hip_check(hipStreamWaitEvent(side, fork_ev[p], 0), "event");
+        part(1, fork_after + 1, 6, st);
+        part(2, 6, ES_STAGES, side);
+        hip_check(hipEventRecord(done_e[p], side), "event");
     }
 
-    // The round on `st`: eager launches, one graph (both chains as branches), or (profiled) one graph per stage with
-    // events around each on its stream.
+    // The round: eager launches, one graph per part, or (profiled) one graph per stage with events around each on its
+    // stream.
     void pipeline(int p, hipStream_t st)
     {
         auto timed = [&](int k, hipStream_t s, auto &&launch) {
@@ -951,25 +971,25 @@ private:
             if (profile) hip_check(hipEventRecord(ev[p][k][1], s), "stage event");
         };
         if (!use_graph) {
-            schedule(st, [&](int k, hipStream_t s) { timed(k, s, [&] { stage(k, p, s); }); });
+            schedule(p, st, [&](int, int k0, int k1, hipStream_t s) {
+                for (int k = k0; k < k1; k++) timed(k, s, [&] { stage(k, p, s); });
+            });
             return;
         }
         if (!profile) {
-            if (!exec[p]) {
-                hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "graph capture");
-                schedule(st, [&](int k, hipStream_t s) { stage(k, p, s); });
-                hip_check(hipStreamEndCapture(st, &graph[p]), "graph capture");
-                hip_check(hipGraphInstantiate(&exec[p], graph[p], nullptr, nullptr, 0), "graph instantiate");
-            }
-            hip_check(hipGraphLaunch(exec[p], st), "graph launch");
+            schedule(p, st, [&](int i, int k0, int k1, hipStream_t s) {
+                if (!pexec[p][i]) pexec[p][i] = capture(k0, k1, p, s, &pgraph[p][i]);
+                hip_check(hipGraphLaunch(pexec[p][i], s), "graph launch");
+            });
             return;
         }
-        schedule(st, [&](int k, hipStream_t s) {
-            timed(k, s, [&] {
-                if (k == 7 && chroma != 1) return;  // no work in the stage: nothing to capture
-                if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, s, &sgraph[p][k]);
-                hip_check(hipGraphLaunch(sexec[p][k], s), "graph launch");
-            });
+        schedule(p, st, [&](int, int k0, int k1, hipStream_t s) {
+            for (int k = k0; k < k1; k++)
+                timed(k, s, [&] {
+                    if (k == 7 && chroma != 1) return;  // no work in the stage: nothing to capture
+                    if (!sexec[p][k]) sexec[p][k] = capture(k, k + 1, p, s, &sgraph[p][k]);
+                    hip_check(hipGraphLaunch(sexec[p][k], s), "graph launch");
+                });
         });
     }
 
@@ -1004,6 +1024,7 @@ private:
     void fold(int p)
     {
         hip_check(hipEventSynchronize(done[p]), "waiting for a round");
+        hip_check(hipEventSynchronize(done_e[p]), "waiting for a round");
         if (profile) {
             for (int k = 0; k < ES_STAGES; k++) {
                 float ms = 0;
