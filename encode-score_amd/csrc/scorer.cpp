@@ -615,8 +615,6 @@ public:
                 if (pexec[k][j]) (void)hipGraphExecDestroy(pexec[k][j]);
                 if (pgraph[k][j]) (void)hipGraphDestroy(pgraph[k][j]);
             }
-            if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
-            if (graph[k]) (void)hipGraphDestroy(graph[k]);
             for (int j = 0; j < ES_STAGES; j++) {
                 if (sexec[k][j]) (void)hipGraphExecDestroy(sexec[k][j]);
                 if (sgraph[k][j]) (void)hipGraphDestroy(sgraph[k][j]);
@@ -719,8 +717,8 @@ private:
     bool profile = false;
     hipStream_t work = nullptr, side = nullptr;  // the round's main stream and the energy chain's
     bool serial = false, early = false;
-    hipGraph_t graph[2] = {}, sgraph[2][ES_STAGES] = {};
-    hipGraphExec_t exec[2] = {}, sexec[2][ES_STAGES] = {};
+    hipGraph_t sgraph[2][ES_STAGES] = {};
+    hipGraphExec_t sexec[2][ES_STAGES] = {};
     hipGraph_t pgraph[2][3] = {};  // graph form, chains overlapped: the work stream's two parts and the side's part
     hipGraphExec_t pexec[2][3] = {};
     Pending pend[2];
