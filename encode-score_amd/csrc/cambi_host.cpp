@@ -115,8 +115,8 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
     hip_check(hipMemsetAsync(counts_.p, 0, counts_.bytes, own_), "clearing counts");
     if (keep_) hip_check(hipMemsetAsync(cvals_.p, 0, cvals_.bytes, own_), "clearing c-values");
     for (int s = 1; s < kScales; s++) launch_cambi_scale(dpic, 1, scale_strips(info_.sw[s], info_.sh[s]), s, own_);
-    // one block per tile, window counts on the matrix cores
-    launch_cambi_cvalues(dpic, first_.as<int>(), kScales, tiles_, tile_h_, 0, nullptr, 0, 32, own_, 1);
+    // one block per tile (window counts on the matrix cores unless set otherwise)
+    launch_cambi_cvalues(dpic, first_.as<int>(), kScales, tiles_, tile_h_, 0, nullptr, 0, bmax_, own_, mfma_ ? 1 : 0);
     for (int pass = 0; pass < 3; pass++) launch_topk_pass(jobs_.as<TopkJob>(), kScales, pass, kTopkBlocks, own_);
     launch_cambi_finish(dpic, jobs_.as<TopkJob>(), 1, score_.as<double>(), own_);
     Results *r = results_.as<Results>();
@@ -125,6 +125,14 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
     hip_check(hipMemcpyAsync(r->counts, counts_.p, counts_.bytes, hipMemcpyDeviceToHost, own_), "reading counts");
     hip_check(hipMemcpyAsync(r->jobs, jobs_.p, jobs_.bytes, hipMemcpyDeviceToHost, own_), "reading the select");
     pending_ = true;
+}
+
+void CambiUnit::set_method(bool mfma, int bmax)
+{
+    require(bmax == 32 || bmax == 64, "the c-value layout holds 32 or 64 numbers per pass");
+    require(!pending_, "method change while a picture is not finished");
+    mfma_ = mfma;
+    bmax_ = bmax;
 }
 
 double CambiUnit::finish()
@@ -195,6 +203,14 @@ int es_cambi_create_ex(int w, int h, int depth, int window_base, int keep_cvalue
 }
 
 int es_cambi_create(int w, int h, int depth, es_cambi **out) { return es_cambi_create_ex(w, h, depth, 65, 0, out); }
+
+int es_cambi_set_method(es_cambi *c, int cv_mfma, int cv_bmax)
+{
+    return guard([&] {
+        require(c != nullptr, "null instance");
+        c->unit.set_method(cv_mfma != 0, cv_bmax);
+    });
+}
 
 int es_cambi_destroy(es_cambi *c)
 {

@@ -19,6 +19,9 @@ public:
     void plane(int s, uint16_t *host) const;
     void cvalues(int s, float *host) const;
     const CambiInfo &info() const { return info_; }
+    // the c-value kernel of the next launches: window counts on the matrix cores (mfma) or by prefix scans, with an
+    // LDS layout of `bmax` (32 or 64) numbers per pass; the results are the same
+    void set_method(bool mfma, int bmax);
 
 private:
     struct Results {
@@ -30,6 +33,8 @@ private:
     CambiInfo info_;
     bool keep_;
     int tile_h_, tiles_ = 0, base_tiles_ = 0;
+    bool mfma_ = true;
+    int bmax_ = 32;
     DevBuf consts_, planes_, cvals_, list_, segs_, sums_, counts_, jobs_, hist_, hsum_, pic_, first_, score_;
     long long loff_[kScales] = {};
     int toff_[kScales] = {};

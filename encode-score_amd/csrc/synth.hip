@@ -4,7 +4,9 @@
 // the sum of twelve 16-bit uniforms less its mean, standard deviation 65536.  Values are carried in 1/256 of a code.
 //
 // kind 0 (config 1): L0 + 30 x / (w - 1) + 30 r / r_max (r from a seeded centre) with grain sigma 2.0 everywhere; the
-// encode (box3) is the noise-free ramp quantised down to 4-code steps, then a 3x3 box mean (edges repeated).
+// encode (box3) is the noise-free ramp quantised down to 4-code steps, then a 3x3 box mean (edges repeated).  L0 is
+// seeded in [300, 480], so the whole ramp lies in CAMBI's band at 10 bits (codes up to tvi[3] = 563) and the encode's
+// steps are measured as banding.
 // kind 1 (configs 2-5): 64x64 tiles of the source's picture, seeded per tile: 40% linear gradients (16 directions,
 // slope 0.05-0.5 codes/px), 40% static band-limited texture (white noise of sigma 29.25 through a 5x5 binomial
 // filter: sigma about 8), 20% dark flat tiles in [64, 120] with grain sigma 1.5 per frame.  An encode keeps a share
@@ -220,7 +222,7 @@ Gen make_gen(const es_synth_desc &d)
     g.block_pct = d.block_pct;
     g.box3 = d.box3;
     if (d.kind == 0) {
-        g.l0 = 300 + (long long)(hash4(d.seed, 1, 0, 0, 0) % 301);
+        g.l0 = 300 + (long long)(hash4(d.seed, 1, 0, 0, 0) % 181);
         g.cx = (long long)(hash4(d.seed, 2, 0, 0, 0) % (uint64_t)d.full_w);
         g.cy = (long long)(hash4(d.seed, 3, 0, 0, 0) % (uint64_t)d.full_h);
         long long rmax = 1;

@@ -123,7 +123,7 @@ for _n, _a in [
     ("es_cambi_destroy", [_vp]), ("es_cambi_launch", [_vp, _vp, _i, _vp]),
     ("es_cambi_launch_ex", [_vp, _vp, _i, _i, _i, _vp]), ("es_cambi_finish", [_vp, _dp]),
     ("es_cambi_scales", [_vp, C.POINTER(CambiScale)]), ("es_cambi_plane", [_vp, _i, _vp]),
-    ("es_cambi_cvalues", [_vp, _i, _vp]),
+    ("es_cambi_cvalues", [_vp, _i, _vp]), ("es_cambi_set_method", [_vp, _i, _i]),
     ("es_window_stats", [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
     ("es_window_values", [_vp, _i, _vp]),
     ("es_persist_stats", [_vp, _sz, _i, _vp, _i, _i, _i, _vp, _vp]),
@@ -289,11 +289,13 @@ def cambi_params(w, h, depth=10, window_base=65):
 class Cambi:
     """The CAMBI object of src/cambi.hpp: launch() then finish() per picture."""
 
-    def __init__(self, w, h, depth=10, window_base=65, keep_cvalues=False):
+    def __init__(self, w, h, depth=10, window_base=65, keep_cvalues=False, cv_mfma=True, cv_bmax=32):
         p = _vp()
         check(lib.es_cambi_create_ex(w, h, depth, window_base, int(keep_cvalues), C.byref(p)))
         self.ptr = p.value
         self.info = cambi_params(w, h, depth, window_base)
+        if not cv_mfma or cv_bmax != 32:
+            check(lib.es_cambi_set_method(self.ptr, int(cv_mfma), cv_bmax))
 
     def launch(self, img, pitch, stream=None, bytes=2, shift=0):
         check(lib.es_cambi_launch_ex(self.ptr, _ptr(img), pitch, bytes, shift, _ptr(stream)))

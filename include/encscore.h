@@ -89,6 +89,10 @@ int es_cambi_create(int w, int h, int depth, es_cambi **out);
  * es_cambi_cvalues when keep_cvalues is nonzero. */
 int es_cambi_create_ex(int w, int h, int depth, int window_base, int keep_cvalues, es_cambi **out);
 int es_cambi_destroy(es_cambi *c);
+/* The c-value kernel of the instance's next launches (no counterpart in the reference, whose cambi_cvalues has one
+ * form, src/kernels.cu:129-278): window counts on the matrix cores (cv_mfma != 0, the default) or by prefix scans,
+ * with an LDS layout of cv_bmax (32, the default, or 64) band values per pass.  Every form gives the same c-values. */
+int es_cambi_set_method(es_cambi *c, int cv_mfma, int cv_bmax);
 /* Cambi::launch(CUdeviceptr img, int pitch, CUstream) (src/cambi.hpp:15-18): img is a device plane of uint16
  * samples with `pitch` in pixels; it is read in stream order on `stream` and the rest runs on the instance's own
  * stream, so the caller may overwrite the plane with work queued on `stream` afterwards. */

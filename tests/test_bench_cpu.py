@@ -1,0 +1,44 @@
+"""bench.py's host logic on the CPU: the frame pool and the autotuner's frames under any --steps / --warmup, and the
+configurations' shapes and lengths."""
+import importlib
+
+import pytest
+
+bench = importlib.import_module("bench")
+
+
+@pytest.mark.parametrize("steps,warmup", [(20, 5), (1, 0), (8, 2), (240, 16), (1000, 50)])
+def test_pool_holds_the_autotuners_frames(steps, warmup):
+    # the autotuner times three batches of its largest batch on the pool's first frames, whatever the run's length
+    pool = bench.frame_pool(steps, warmup, tune=1, batch=32, cap=256)
+    assert pool >= bench.TUNE_FRAMES >= 3 * bench.TUNE_BATCH_MAX
+    assert pool <= max(256, bench.TUNE_FRAMES)
+
+
+def test_pool_covers_a_short_run_without_reuse():
+    # a run shorter than the cap gets a distinct frame for every frame step
+    assert bench.frame_pool(3, 1, tune=1, batch=32, cap=256) == 4 * 32
+    assert bench.frame_pool(3, 1, tune=0, batch=8, cap=256) == bench.TUNE_FRAMES
+    assert bench.frame_pool(30, 2, tune=0, batch=8, cap=512) == 256
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("total", [96, 160, 800, 1440])
+def test_configuration_lengths(es, n, total):
+    src, encs, sbytes, shift, text = bench.config(n, es, total)
+    lengths = [length for _, length in encs]
+    # the longest encode runs the whole run, none runs past it, and every one reaches the lasting window (13 frames)
+    assert max(lengths) == total
+    assert all(13 <= length <= total for length in lengths)
+    assert lengths == sorted(lengths, reverse=True)
+    for d, _ in encs:
+        assert d.w % 2 == 0 and d.h % 2 == 0 and d.w <= src.w and d.h <= src.h
+    assert (sbytes, shift) == ((1, 2) if n == 4 else (2, 0))
+    assert text.startswith(f"C{n}")
+
+
+def test_config3_and_5_keep_the_reference_proportions(es):
+    _, encs, _, _, _ = bench.config(3, es, 1440)
+    assert [length for _, length in encs] == [1440, 1100, 720, 333]
+    _, encs, _, _, _ = bench.config(5, es, 1440)
+    assert [length for _, length in encs][0] == 1440 and [length for _, length in encs][-1] == 500
