@@ -142,19 +142,20 @@ __device__ __forceinline__ void block_sum_ll(long long (&v)[N], long long (&part
     for (int k = 0; k < N; k++) v[k] = part[k][0] + part[k][1] + part[k][2] + part[k][3];
 }
 
-// Keys for the structure threshold: the bit patterns of the source's oriented energies (non-negative doubles, which
-// order like their values).
+// Keys for the structure threshold: the bit patterns of the source's oriented energies rounded once to float (the
+// reference's per-window figures are floats; non-negative floats order like their bit patterns).
 __global__ __launch_bounds__(kCollectThreads) void collect_keys_kernel(const CollectJob *jobs, CollectWork w)
 {
     const CollectJob &J = jobs[blockIdx.y];
     const int i = blockIdx.x * kCollectThreads + threadIdx.x;
     if (i < J.windows)
         w.keys[(size_t)blockIdx.y * w.max_windows + i] =
-            (unsigned long long)__double_as_longlong(oriented_r(J.sums + (size_t)i * 12));
+            (unsigned long long)__float_as_uint(__double2float_rn(oriented_r(J.sums + (size_t)i * 12)));
 }
 
-// numpy's linear quantile at 0.8 of a job's keys (src/main.cpp:279-289): the key of rank lo = floor(0.8 (n - 1)) by
-// 8 passes of 8-bit digits, each pass counting into per-wave histograms, and the next larger key for lo + 1.
+// numpy's linear quantile at 0.8 of a job's keys as the reference evaluates it (src/main.cpp:279-289): the key a of
+// rank lo = floor(0.8 (n - 1)) by 4 passes of 8-bit digits, each pass counting into per-wave histograms, and the next
+// larger key b for lo + 1; then b - a in float, a + (b - a)(pos - lo) in double, rounded to float.
 __global__ __launch_bounds__(kSelectThreads) void collect_select_kernel(const CollectJob *jobs, CollectWork w)
 {
     constexpr int kWaves = kSelectThreads / 64;
@@ -170,13 +171,13 @@ __global__ __launch_bounds__(kSelectThreads) void collect_select_kernel(const Co
     const long long lo = (long long)pos;
     unsigned long long prefix = 0, r = (unsigned long long)lo;
     unsigned eq = 0;
-    for (int shift = 56; shift >= 0; shift -= 8) {
+    for (int shift = 24; shift >= 0; shift -= 8) {
         for (int i = tid & 63; i < 256; i += 64) hist[wave][i] = 0;
         __syncthreads();
 #pragma unroll 4
         for (long long i = tid; i < n; i += kSelectThreads) {
             const unsigned long long key = keys[i];
-            if (shift == 56 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
+            if (shift == 24 || (key >> (shift + 8)) == (prefix >> (shift + 8)))
                 atomicAdd(&hist[wave][(unsigned)(key >> shift) & 255u], 1u);
         }
         __syncthreads();
@@ -193,12 +194,13 @@ __global__ __launch_bounds__(kSelectThreads) void collect_select_kernel(const Co
         prefix |= (unsigned long long)d << shift;
         __syncthreads();
     }
-    double q = __longlong_as_double((long long)prefix);
+    const float a = __uint_as_float((unsigned)prefix);
+    float q = a;
     if (lo + 1 < n) {
         unsigned long long kb = prefix;
         if (r + 1 >= eq) kb = block_min_above(keys, n, prefix);  // no second copy of the key at rank lo + 1
-        const double a = q, b = __longlong_as_double((long long)kb);
-        q = __dadd_rn(a, __dmul_rn(b - a, pos - (double)lo));
+        const float b = __uint_as_float((unsigned)kb);
+        q = __double2float_rn(__dadd_rn((double)a, __dmul_rn((double)__fsub_rn(b, a), pos - (double)lo)));
     }
     if (tid == 0) *J.quantile = q;
 }
@@ -213,7 +215,7 @@ __global__ __launch_bounds__(kCollectThreads) void collect_apply_kernel(const Co
     const CollectJob &J = jobs[job];
     const int n = J.windows, nchunks = (n + kCollectThreads - 1) / kCollectThreads;
     if (chunk >= nchunks) return;  // uniform over the block
-    const double q = *J.quantile;
+    const float q = (float)*J.quantile;  // a float value held in a double
     const int i = chunk * kCollectThreads + tid;
     long long zi[12] = {0};
     double o[4] = {0, 0, 0, 0};
@@ -226,7 +228,8 @@ __global__ __launch_bounds__(kCollectThreads) void collect_apply_kernel(const Co
         for (int k = 0; k < 12; k++) zz[k] = z[k];
         double v[8];
         window_values(zz, v);
-        m = v[6] >= q && v[6] > 0;
+        const float of = __double2float_rn(v[6]);  // compared as the reference's float figure
+        m = of >= q && of > 0.0f;
         J.mask[i] = (uint8_t)m;
 #pragma unroll
         for (int k = 0; k < 6; k++) {

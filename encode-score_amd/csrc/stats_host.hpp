@@ -210,21 +210,29 @@ inline void window_values_host(const long long *z, double *v)
 }
 
 // The frame part of collect on the host (src/main.cpp:606-629): numpy's linear 0.8 quantile of the oriented
-// energies by nth_element (src/main.cpp:279-289), the mask, histogram entries and the canonical frame totals.
+// energies as the reference evaluates it (src/main.cpp:279-289: float keys, the difference of the two order
+// statistics in float, the interpolation in double rounded to float), the mask (each window's oriented energy
+// rounded once to float against it), histogram entries and the canonical frame totals.
 inline double collect_frame_host(Accum &acc, const long long *sums, int windows, uint8_t *mask, long long *ti, double *to)
 {
-    std::vector<double> v((size_t)windows * 8), orient(windows);
+    std::vector<double> v((size_t)windows * 8);
+    std::vector<float> orient(windows), keys;
     for (int i = 0; i < windows; i++) {
         window_values_host(sums + (size_t)i * 12, &v[(size_t)i * 8]);
-        orient[i] = v[(size_t)i * 8 + 6];
+        orient[i] = (float)v[(size_t)i * 8 + 6];
     }
+    keys = orient;
     const double pos = 0.8 * (double)(windows - 1);
     const size_t lo = (size_t)pos;
-    std::nth_element(orient.begin(), orient.begin() + lo, orient.end());
-    double q = orient[lo];
-    if (lo + 1 < orient.size()) {
-        const double a = q, b = *std::min_element(orient.begin() + lo + 1, orient.end());
-        q = a + (b - a) * (pos - (double)lo);
+    float q = 0.0f;
+    if (windows > 0) {
+        std::nth_element(keys.begin(), keys.begin() + lo, keys.end());
+        q = keys[lo];
+        if (lo + 1 < keys.size()) {
+            const float a = q, b = *std::min_element(keys.begin() + lo + 1, keys.end());
+            const float d = b - a;
+            q = (float)((double)a + (double)d * (pos - (double)lo));
+        }
     }
     for (int k = 0; k < 12; k++) ti[k] = 0;
     double tree[4][256];
@@ -236,7 +244,7 @@ inline double collect_frame_host(Accum &acc, const long long *sums, int windows,
             if (i < windows) {
                 const double *s = &v[(size_t)i * 8];
                 const long long *z = sums + (size_t)i * 12;
-                const bool m = s[6] >= q && s[6] > 0;
+                const bool m = orient[i] >= q && orient[i] > 0.0f;
                 mask[i] = m;
                 for (int k = 0; k < 6; k++) {
                     ti[k] += z[k];

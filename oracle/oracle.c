@@ -573,8 +573,9 @@ static int cmp_double(const void *a, const void *b)
     return x < y ? -1 : x > y ? 1 : 0;
 }
 
-/* quantile (src/main.cpp:279-289): numpy's linear interpolation between the order statistics of ranks
- * floor(q (n - 1)) and the one after it, here in FP64 throughout. */
+/* numpy's linear quantile (Hyndman-Fan type 7, the definition src/main.cpp:279-289 restates): interpolation between
+ * the order statistics of ranks floor(q (n - 1)) and the one after it, in FP64 throughout.  The collect uses
+ * orc_quantile_f32, the reference's float evaluation of the same definition. */
 double orc_quantile_linear(const double *v, long long n, double q)
 {
     double *c = (double *)malloc((size_t)n * sizeof(double));
@@ -587,6 +588,34 @@ double orc_quantile_linear(const double *v, long long n, double q)
     if (lo + 1 < n) {
         const double b = c[lo + 1];
         out = a + (b - a) * (pos - (double)lo);
+    }
+    free(c);
+    return out;
+}
+
+static int cmp_float(const void *a, const void *b)
+{
+    const float x = *(const float *)a, y = *(const float *)b;
+    return x < y ? -1 : x > y ? 1 : 0;
+}
+
+/* quantile (src/main.cpp:279-289) as the reference evaluates it: float keys; a = the key of rank lo = floor(q (n - 1))
+ * and b = the smallest key after it; b - a is a float difference, a + (b - a)(pos - lo) is evaluated in double and
+ * rounded to float. */
+float orc_quantile_f32(const float *v, long long n, double q)
+{
+    if (n <= 0) return 0.0f;
+    float *c = (float *)malloc((size_t)n * sizeof(float));
+    memcpy(c, v, (size_t)n * sizeof(float));
+    qsort(c, (size_t)n, sizeof(float), cmp_float);
+    const double pos = q * (double)(n - 1);
+    const long long lo = (long long)pos;
+    const float a = c[lo];
+    float out = a;
+    if (lo + 1 < n) {
+        const float b = c[lo + 1];
+        const float d = b - a;
+        out = (float)((double)a + (double)d * (pos - (double)lo));
     }
     free(c);
     return out;
@@ -718,25 +747,27 @@ void orc_accum_score(const orc_accum *a, double vmaf_total, orc_parts *p)
 }
 
 /* The per-frame part of collect (src/main.cpp:606-629): the structure mask (oriented >= its 0.8 quantile and > 0),
- * the frame's totals and the per-window histogram entries.  Canonical totals: fine and mid (and their shared parts)
- * are the exact integer sums over the windows scaled once; oriented is orc_tree_sum256 of the per-window values (0
- * for windows outside the mask in the structured totals). */
+ * the frame's totals and the per-window histogram entries.  The mask compares floats as the reference does: each
+ * window's oriented energy rounded once to float (the reference's per-window figures are floats) against
+ * orc_quantile_f32 of them.  Canonical totals: fine and mid (and their shared parts) are the exact integer sums over
+ * the windows scaled once; oriented is orc_tree_sum256 of the per-window values (0 for windows outside the mask in the
+ * structured totals). */
 void orc_collect_frame(orc_accum *acc, const long long *sums, int windows, uint8_t *mask_out)
 {
     double *st = (double *)malloc((size_t)windows * 8 * sizeof(double));
-    double *orient = (double *)malloc((size_t)windows * sizeof(double));
+    float *orient = (float *)calloc((size_t)windows + 1, sizeof(float));
     double *ob[4];
     for (int k = 0; k < 4; k++) ob[k] = (double *)malloc((size_t)windows * sizeof(double));
     for (int i = 0; i < windows; i++) {
         orc_window_values(sums + (size_t)i * 12, st + (size_t)i * 8);
-        orient[i] = st[(size_t)i * 8 + 6];
+        orient[i] = (float)st[(size_t)i * 8 + 6];
     }
-    const double q = orc_quantile_linear(orient, windows, 0.8);
+    const float q = orc_quantile_f32(orient, windows, 0.8);
     long long all[6] = {0}, stc[6] = {0};
     for (int i = 0; i < windows; i++) {
         const double *s = st + (size_t)i * 8;
         const long long *z = sums + (size_t)i * 12;
-        const int m = s[6] >= q && s[6] > 0;
+        const int m = orient[i] >= q && orient[i] > 0;
         mask_out[i] = (uint8_t)m;
         for (int j = 0; j < 6; j++) {
             all[j] += z[j];

@@ -78,6 +78,7 @@ void orc_persist_values(const long long *sums, double *vals);
 
 /* Selects and the histogram (src/main.cpp:279-289, src/stats.hpp:8-32). */
 double orc_quantile_linear(const double *v, long long n, double q);
+float orc_quantile_f32(const float *v, long long n, double q);
 unsigned long long orc_select_rank_u64(const unsigned long long *keys, long long n, long long rank);
 double orc_tree_sum256(const double *v, long long n);
 

@@ -92,6 +92,7 @@ _sig("orc_window_values", [_vp, _vp])
 _sig("orc_persist", [_vp, _vp, _i, _i, _i, _vp])
 _sig("orc_persist_values", [_vp, _vp])
 _sig("orc_quantile_linear", [_vp, _ll, _dp], _dp)
+_sig("orc_quantile_f32", [_vp, _ll, _dp], C.c_float)
 _sig("orc_select_rank_u64", [_vp, _ll, _ll], C.c_ulonglong)
 _sig("orc_tree_sum256", [_vp, _ll], _dp)
 _sig("orc_hist_bin", [_dp], _i)
@@ -223,6 +224,12 @@ def persist(ring, mid_d, win):
 def quantile_linear(v, q):
     v = np.ascontiguousarray(v, np.float64)
     return lib.orc_quantile_linear(v.ctypes.data, v.size, q)
+
+
+def quantile_f32(v, q):
+    """The structure threshold as the reference evaluates it (src/main.cpp:279-289) on float keys."""
+    v = np.ascontiguousarray(v, np.float32)
+    return lib.orc_quantile_f32(v.ctypes.data, v.size, q)
 
 
 def tree_sum256(v):

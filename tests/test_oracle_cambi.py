@@ -70,6 +70,75 @@ def test_band_and_visibility_threshold(orc):
     assert list(p.weight) == [1, 2, 3, 4]
 
 
+def test_bt1886_defining_identities(orc):
+    """BT.1886 fixes a and b by L(V = 0) = black and L(V = 1) = white; the codes below the limited range's foot and
+    above its head are clipped to it; luminance rises strictly over the range."""
+    L = np.array([orc.lib.orc_bt1886_luminance(s) for s in range(1024)])
+    assert L[64] == pytest.approx(0.01, rel=1e-12) and L[940] == pytest.approx(300.0, rel=1e-12)
+    assert (L[:64] == L[64]).all() and (L[940:] == L[940]).all()
+    assert (np.diff(L[64:941]) > 0).all()
+    # the exponent: between the ends, L^(1/2.4) is affine in the code
+    r = L[64:941] ** (1 / 2.4)
+    assert np.allclose(np.diff(r), (r[-1] - r[0]) / 876, rtol=1e-9)
+
+
+def tvi_bisection(lum, diff, thr, foot=64, head=940):
+    """libvmaf's get_tvi_for_diff as published: bisection over the luma range for the code whose `diff` step is a
+    visible rise (delta L > thr * L) while the step from one code higher is not."""
+    cond = lambda s: lum(s + diff) - lum(s) > thr * lum(s)
+    lo, hi = foot, head
+    while True:
+        mid = lo + (hi - lo) // 2
+        if not cond(mid):
+            hi = mid
+        elif cond(mid + 1):
+            lo = mid
+        else:
+            return mid
+
+
+@pytest.mark.parametrize("thr", [0.015, 0.019, 0.025, 0.03, 0.05])
+@pytest.mark.parametrize("diff", [1, 2, 3, 4])
+def test_tvi_bisection_equals_the_scan(orc, diff, thr):
+    """The visibility condition holds on a prefix of the range and fails after it (the relative step of a power law
+    falls with the level), so the bisection's answer is unique and equals the oracle's scan; it lies inside the range,
+    away from the ends where variants of the search could differ."""
+    lum = orc.lib.orc_bt1886_luminance
+    t = orc.lib.orc_tvi_for_diff(diff, thr)
+    assert tvi_bisection(lum, diff, thr) == t
+    cond = np.array([lum(s + diff) - lum(s) > thr * lum(s) for s in range(64, 941)])
+    k = t - 64 + 1
+    assert cond[:k].all() and not cond[k:].any()
+    assert 64 < t < 940 - diff
+
+
+def test_tvi_values_at_the_reference_threshold(orc):
+    """tvi + 4 for diffs 1..4 at 1.9% (src/cambi.cpp:46-49): the band every configuration uses."""
+    assert list(orc.params(1920, 1080).tvi) == [182, 309, 436, 563]
+
+
+@pytest.mark.parametrize("vlt", [0, 1, 11, 40, 63, 64])
+def test_visibility_threshold_reading_leaves_limited_range_results_alone(es, orc, vlt):
+    """get_vlt_luma(0.0) (src/cambi.cpp:50): every code's luminance is at least the black level (0.01 cd/m^2) > 0, so code 0
+    already reaches the threshold; any reading of the search that gives a code up to the foot (64) derives
+    band_base = max(vlt - 11, 0) <= 53 (src/cambi.cpp:51-52) and passes value + d + 1 > vlt for every sample of the
+    limited range: the c-values and the score of limited-range pictures do not depend on it."""
+    assert min(orc.lib.orc_bt1886_luminance(s) for s in range(1024)) > 0.0
+    for img in (banded_mosaic(es, 160, 96, 8, quant_step=4, quant_where=1), staircase(48, 120, base=64)):
+        h, w = img.shape
+        assert img.min() >= 64
+        p0 = orc.params(w, h)
+        ref, ref_cv = orc.cambi(p0, img, want_cvalues=True)
+        p = orc.params(w, h)
+        p.vlt = vlt
+        p.band_base = max(vlt - 11, 0)
+        p.band_size = p.tvi[3] + 1 - p.band_base
+        got, got_cv = orc.cambi(p, img, want_cvalues=True)
+        assert got.score == ref.score
+        for s in range(5):
+            assert np.array_equal(got_cv[s], ref_cv[s])
+
+
 def mask_numpy(img, index):
     h, w = img.shape
     a = img.astype(np.int64)

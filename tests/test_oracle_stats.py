@@ -79,6 +79,43 @@ def test_quantile_is_numpys_linear(orc, n):
     assert orc.quantile_linear(v, 0.8) == pytest.approx(np.quantile(v, 0.8), rel=1e-14, abs=1e-14)
 
 
+@pytest.mark.parametrize("n", [2, 7, 100, 8040, 32400])
+def test_quantile_f32_rounds_numpys_linear_quantile(orc, n):
+    """On float keys whose neighbouring order statistics are within a factor 2 (b - a is then exact in float), the
+    reference's evaluation is numpy's linear quantile rounded once to float."""
+    v = np.random.default_rng(n).gamma(4.0, 300.0, n).astype(np.float32)
+    c = np.sort(v)
+    lo = int(0.8 * (n - 1))
+    if lo + 1 < n:
+        assert c[lo] >= c[lo + 1] / 2
+    assert orc.quantile_f32(v, 0.8) == np.float32(np.quantile(v.astype(np.float64), 0.8))
+
+
+def test_quantile_f32_near_tie():
+    """Keys one float ulp apart (src/main.cpp:288 rounds the interpolated value to float): with 9 keys the threshold
+    sits 0.4 ulp above the rank-6 key and rounds back onto it, so that window is in the mask; kept in double it would
+    not be."""
+    orc = __import__("conftest").load_oracle()
+    one, next_up = np.float32(1.0), np.nextafter(np.float32(1.0), np.float32(2.0))
+    v = np.array([0.1, 0.2, 0.3, 0.4, 0.5, 0.6, one, next_up, 2.0], np.float32)
+    q = orc.quantile_f32(v, 0.8)
+    assert q == 1.0
+    assert int(((v >= q) & (v > 0)).sum()) == 3
+    assert orc.quantile_linear(v.astype(np.float64), 0.8) > 1.0  # the FP64 threshold would leave it out
+    # 12 keys: 0.8 ulp above, rounded up onto the next key
+    v = np.array([0.1] * 8 + [one, next_up, 3.0, 4.0], np.float32)
+    assert orc.quantile_f32(v, 0.8) == next_up
+
+
+def test_quantile_f32_difference_in_float():
+    """b - a is a float difference: 2^25 - 1 rounds to 2^25 before the interpolation (q = 1 + 0.4 * 2^25 rounded
+    to float), where an exact difference would give 13421773."""
+    orc = __import__("conftest").load_oracle()
+    v = np.array([0.0] * 6 + [1.0, 2.0 ** 25, 2.0 ** 26], np.float32)
+    assert orc.quantile_f32(v, 0.8) == np.float32(13421774.0)
+    assert np.float32(1.0 + (2.0 ** 25 - 1.0) * 0.4) == np.float32(13421773.0)
+
+
 def test_structure_mask_picks_the_top_windows(orc):
     """tests/test_reference_compare.py:59-68: the windows at or above the quantile and above zero."""
     grid = np.zeros((10, 10))
