@@ -1,107 +1,149 @@
-// CambiCacheFile: the on-disk source-CAMBI cache (src/cache.cpp:14-88).  The file format is the reference's: an
-// append-only sequence of 16-byte records {int64 pts, double score} in host byte order, named by a 64-bit FNV-1a
-// hash (16 hex digits, ".cambi") of the key "cambi <version> <source size> <x0> <y0> <w> <h> <depth>" followed by the
-// source's first and last mebibyte.  A record cut short by an interrupted run is dropped when the file is opened.
+// SourceCambiStore (see cache.hpp).  On disk (the reference's format, src/cache.cpp:14-88): an append-only sequence of
+// 16-byte records {int64 pts, double score} in host byte order, in a file named by the 64-bit FNV-1a hash of the key
+// "cambi <version> <source size> <x0> <y0> <w> <h> <depth>" continued over the source's first and then last MiB (the
+// whole file twice when it is shorter), as 16 hex digits and ".cambi".  A record cut short by an interrupted run is
+// cut off when the store is opened; later records of the same timestamp replace earlier ones.
 #include "cache.hpp"
 #include "host_common.hpp"
 
-#include <cstdio>
 #include <cstdlib>
-#include <fcntl.h>
-#include <sys/stat.h>
-#include <unistd.h>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+
+namespace fs = std::filesystem;
 
 namespace es {
 
-uint64_t CambiCacheFile::fnv1a(uint64_t h, const void *data, size_t n)
+void Fnv1a64::update(const void *data, size_t n)
 {
-    const unsigned char *p = (const unsigned char *)data;
-    for (size_t i = 0; i < n; i++) {
-        h ^= p[i];
-        h *= 0x100000001b3ull;  // the 64-bit FNV prime
-    }
-    return h;
+    const unsigned char *p = static_cast<const unsigned char *>(data);
+    for (const unsigned char *end = p + n; p != end; ++p) h_ = (h_ ^ *p) * 0x100000001b3ull;
 }
 
-CambiCacheFile::CambiCacheFile(const std::string &path, int x0, int y0, int w, int h, int depth, const std::string &dir_in)
+namespace {
+
+// The store's directory: `dir` as given, or the user's cache directory with "encode-score" under it; created when
+// missing.  Empty when no directory can be named or made.
+std::string store_dir(const std::string &dir)
 {
-    std::string dir = dir_in;
-    if (dir.empty()) {
+    fs::path d;
+    if (!dir.empty()) {
+        d = dir;
+    } else {
         const char *xdg = std::getenv("XDG_CACHE_HOME"), *home = std::getenv("HOME");
-        if (xdg && *xdg) dir = xdg;
-        else if (home) dir = std::string(home) + "/.cache";
-        else return;
-        mkdir(dir.c_str(), 0755);
-        dir += "/encode-score";
+        if (xdg && *xdg) d = fs::path(xdg) / "encode-score";
+        else if (home && *home) d = fs::path(home) / ".cache" / "encode-score";
+        else return "";
     }
-    mkdir(dir.c_str(), 0755);
-    FILE *src = std::fopen(path.c_str(), "rb");
-    if (!src) return;
-    std::fseek(src, 0, SEEK_END);
-    const long size = std::ftell(src);
-    char key[112];
-    const int len = std::snprintf(key, sizeof key, "cambi %d %ld %d %d %d %d %d", kVersion, size, x0, y0, w, h, depth);
-    uint64_t hash = fnv1a(0xcbf29ce484222325ull, key, (size_t)len);  // the 64-bit FNV offset basis
-    std::vector<unsigned char> buf(kSample);
-    const long last = size > (long)kSample ? size - (long)kSample : 0;
-    for (long at : {0L, last}) {
-        std::fseek(src, at, SEEK_SET);
-        const size_t got = std::fread(buf.data(), 1, kSample, src);
-        hash = fnv1a(hash, buf.data(), got);
-    }
-    std::fclose(src);
-    char name[40];
-    std::snprintf(name, sizeof name, "/%016llx.cambi", (unsigned long long)hash);
-    file_ = dir + name;
-    fd_ = open(file_.c_str(), O_RDWR | O_CREAT | O_APPEND, 0644);
-    if (fd_ < 0) return;
-    struct stat st;
-    if (fstat(fd_, &st) != 0) {
-        close(fd_);
-        fd_ = -1;
-        return;
-    }
-    const off_t whole = st.st_size - st.st_size % (off_t)sizeof(Record);
-    if (whole != st.st_size && ftruncate(fd_, whole) != 0) {
-        close(fd_);
-        fd_ = -1;
-        return;
-    }
-    std::vector<Record> recs((size_t)whole / sizeof(Record));
-    if (whole > 0 && pread(fd_, recs.data(), (size_t)whole, 0) == (ssize_t)whole)
-        for (const Record &r : recs) scores_[r.pts] = r.score;
+    std::error_code ec;
+    fs::create_directories(d, ec);
+    return fs::is_directory(d, ec) ? d.string() : "";
 }
 
-CambiCacheFile::~CambiCacheFile()
+// Hashes `n` bytes of `in` from `offset` (fewer when the file ends first).
+void hash_span(std::ifstream &in, uint64_t offset, size_t n, Fnv1a64 &h)
 {
-    flush();
-    if (fd_ >= 0) close(fd_);
+    std::vector<char> buf(n);
+    in.clear();
+    in.seekg((std::streamoff)offset);
+    in.read(buf.data(), (std::streamsize)n);
+    h.update(buf.data(), (size_t)in.gcount());
 }
 
-bool CambiCacheFile::find(int64_t pts, double *score) const
+}  // namespace
+
+std::string SourceCambiStore::name_of(const std::string &path, int x0, int y0, int w, int h, int depth)
 {
-    if (pts == INT64_MIN) return false;  // no timestamp (AV_NOPTS_VALUE)
-    const auto it = scores_.find(pts);
-    if (it == scores_.end()) return false;
+    std::error_code ec;
+    const uintmax_t size = fs::file_size(path, ec);
+    std::ifstream in(path, std::ios::binary);
+    if (ec || !in) return "";
+    const std::string key = "cambi " + std::to_string(kVersion) + " " + std::to_string(size) + " " + std::to_string(x0) +
+                            " " + std::to_string(y0) + " " + std::to_string(w) + " " + std::to_string(h) + " " +
+                            std::to_string(depth);
+    Fnv1a64 hash;
+    hash.update(key.data(), key.size());
+    hash_span(in, 0, kSampleBytes, hash);
+    hash_span(in, size > kSampleBytes ? size - kSampleBytes : 0, kSampleBytes, hash);
+    static const char digits[] = "0123456789abcdef";
+    std::string name(16, '0');
+    for (int i = 0; i < 16; i++) name[i] = digits[(hash.value() >> (60 - 4 * i)) & 15];
+    return name + ".cambi";
+}
+
+SourceCambiStore::SourceCambiStore(const std::string &path, int x0, int y0, int w, int h, int depth, const std::string &dir)
+{
+    const std::string where = store_dir(dir);
+    const std::string name = where.empty() ? "" : name_of(path, x0, y0, w, h, depth);
+    if (name.empty()) return;  // inert
+    file_ = (fs::path(where) / name).string();
+    load();
+}
+
+// Reads the whole records of the file (cutting a partial last one off it).
+void SourceCambiStore::load()
+{
+    std::error_code ec;
+    if (!fs::exists(file_, ec)) return;
+    const uintmax_t size = fs::file_size(file_, ec);
+    if (ec) {
+        file_.clear();
+        return;
+    }
+    const uintmax_t whole = size - size % kRecordBytes;
+    if (whole != size) {
+        fs::resize_file(file_, whole, ec);
+        if (ec) {
+            file_.clear();
+            return;
+        }
+    }
+    std::ifstream in(file_, std::ios::binary);
+    std::vector<unsigned char> raw((size_t)whole);
+    if (!in.read(reinterpret_cast<char *>(raw.data()), (std::streamsize)whole)) {
+        file_.clear();
+        return;
+    }
+    for (size_t at = 0; at < raw.size(); at += kRecordBytes) {
+        int64_t pts;
+        double score;
+        std::memcpy(&pts, &raw[at], 8);
+        std::memcpy(&score, &raw[at + 8], 8);
+        known_[pts] = score;
+    }
+}
+
+SourceCambiStore::~SourceCambiStore() { write_pending(); }
+
+bool SourceCambiStore::lookup(int64_t pts, double *score) const
+{
+    if (pts == INT64_MIN) return false;  // a frame without a timestamp (AV_NOPTS_VALUE)
+    const auto it = known_.find(pts);
+    if (it == known_.end()) return false;
     *score = it->second;
     return true;
 }
 
-void CambiCacheFile::add(int64_t pts, double score)
+void SourceCambiStore::keep(int64_t pts, double score)
 {
-    if (fd_ < 0 || pts == INT64_MIN) return;
-    scores_[pts] = score;
-    pending_.push_back(Record{pts, score});
-    if (pending_.size() >= kFlush) flush();
+    if (file_.empty() || pts == INT64_MIN) return;
+    known_[pts] = score;
+    unsigned char rec[kRecordBytes];
+    std::memcpy(rec, &pts, 8);
+    std::memcpy(rec + 8, &score, 8);
+    unwritten_.insert(unwritten_.end(), rec, rec + kRecordBytes);
+    if (unwritten_.size() >= kBatch * kRecordBytes) write_pending();
 }
 
-void CambiCacheFile::flush()
+void SourceCambiStore::write_pending()
 {
-    if (fd_ >= 0 && !pending_.empty() && write(fd_, pending_.data(), pending_.size() * sizeof(Record)) < 0) {
-        close(fd_);
-        fd_ = -1;
-    }
-    pending_.clear();
+    if (file_.empty() || unwritten_.empty()) return;
+    std::ofstream out(file_, std::ios::binary | std::ios::app);
+    out.write(reinterpret_cast<const char *>(unwritten_.data()), (std::streamsize)unwritten_.size());
+    out.flush();
+    if (!out) file_.clear();  // the file cannot be written: keep nothing more
+    unwritten_.clear();
 }
 
 }  // namespace es
@@ -109,9 +151,9 @@ void CambiCacheFile::flush()
 using namespace es;
 
 struct es_cambi_cache_s {
-    CambiCacheFile f;
+    SourceCambiStore store;
     es_cambi_cache_s(const char *path, int x0, int y0, int w, int h, int depth, const char *dir)
-        : f(path, x0, y0, w, h, depth, dir ? dir : "") {}
+        : store(path, x0, y0, w, h, depth, dir ? dir : "") {}
 };
 
 extern "C" {
@@ -128,7 +170,7 @@ int es_cambi_cache_find(const es_cambi_cache *c, int64_t pts, double *score, int
 {
     return guard([&] {
         require(c && score && found, "null argument");
-        *found = c->f.find(pts, score) ? 1 : 0;
+        *found = c->store.lookup(pts, score) ? 1 : 0;
     });
 }
 
@@ -136,11 +178,11 @@ int es_cambi_cache_add(es_cambi_cache *c, int64_t pts, double score)
 {
     return guard([&] {
         require(c != nullptr, "null cache");
-        c->f.add(pts, score);
+        c->store.keep(pts, score);
     });
 }
 
-const char *es_cambi_cache_file(const es_cambi_cache *c) { return c ? c->f.file().c_str() : ""; }
+const char *es_cambi_cache_file(const es_cambi_cache *c) { return c ? c->store.file().c_str() : ""; }
 
 int es_cambi_cache_close(es_cambi_cache *c)
 {

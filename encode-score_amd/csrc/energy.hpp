@@ -12,6 +12,10 @@ struct WsPair {
     int w, h, up;
     int32_t *mid_r, *mid_d;
     long long *sums;
+    // FP64 form only (any may be null): mid = M / 81 as float planes, and the reference's 8 figures per window as
+    // floats (src/kernels.cu:340-347)
+    float *midf_r, *midf_d;
+    float *vals;
 };
 
 // One lasting-texture job: ring[j] is the source mid plane of offset j - 6 from the centre frame.
@@ -42,7 +46,17 @@ struct LastingJob {
     int centre0, ncentres, slot0;
 };
 
+// The reference-shaped lasting texture on float mid planes (16 x 16 windows): ring[j] is the source's plane of offset
+// j - 6 from the centre frame; out receives 5 floats per window.
+struct PersistF32Job {
+    const float *ring[kRing];
+    const float *mid_d;
+    int w, h;
+    float *out;
+};
+
 void launch_window_stats(const WsPair *pairs, int npairs, int max_w, int max_h, int win, bool fp64, hipStream_t st);
+void launch_persist_f32(const PersistF32Job *jobs, int njobs, int max_w, int max_h, hipStream_t st);
 void launch_lasting(const LastingJob *jobs, int njobs, int max_w, int max_h, int win, hipStream_t st);
 void launch_persist(const PersistJob *jobs, int njobs, int max_w, int max_h, int win, int kind, hipStream_t st);
 

@@ -45,6 +45,49 @@ int es_window_stats(const uint16_t *yr, const uint16_t *yd, int up, int w, int h
     });
 }
 
+int es_window_stats_f32(const uint16_t *yr, const uint16_t *yd, int up, int w, int h, float *mid_r, float *mid_d,
+                        float *out, es_stream s)
+{
+    return guard([&] {
+        ensure_init();
+        require(yr && yd && out, "null pointer");
+        require(w >= 16 && h >= 16, "picture smaller than one window");
+        require(up >= 0 && up <= 2, "up must be 0 to 2");
+        WsPair p{};
+        p.r = PlaneIn{yr, w, 2, 0, 0};
+        p.d = PlaneIn{yd, w, 2, 0, 0};
+        p.w = w;
+        p.h = h;
+        p.up = up;
+        p.midf_r = mid_r;
+        p.midf_d = mid_d;
+        p.vals = out;
+        const auto *d = (const WsPair *)arena().stage(&p, sizeof p, hs(s));
+        launch_window_stats(d, 1, w, h, 16, true, hs(s));
+        arena().done(hs(s));
+    });
+}
+
+int es_persist_stats_f32(const float *ring, size_t plane, int centre, const float *mid_d, int w, int h, float *out,
+                         es_stream s)
+{
+    return guard([&] {
+        ensure_init();
+        require(ring && mid_d && out, "null pointer");
+        require(centre >= 0 && centre < kRing, "centre slot must be 0 to 12");
+        require(plane >= (size_t)w * h && w >= 16 && h >= 16, "bad plane size");
+        PersistF32Job j{};
+        for (int o = -kRadius; o <= kRadius; o++) j.ring[o + kRadius] = ring + (size_t)((centre + o + kRing) % kRing) * plane;
+        j.mid_d = mid_d;
+        j.w = w;
+        j.h = h;
+        j.out = out;
+        const auto *d = (const PersistF32Job *)arena().stage(&j, sizeof j, hs(s));
+        launch_persist_f32(d, 1, w, h, hs(s));
+        arena().done(hs(s));
+    });
+}
+
 int es_window_values(const int64_t *sums, int windows, double *v)
 {
     return guard([&] {

@@ -5,11 +5,8 @@
 // structure tensor of box3's central differences.  Luma is an integer and the box weights are 1/9 and 1/81, so the
 // kernel works on exact integers F = 9 fine, M = 81 mid, GX = 9 gx, GY = 9 gy and writes exact window sums:
 //   [F_r^2, M_r^2, F_d^2, M_d^2, F_r F_d, M_r M_d, GX_r^2, GY_r^2, GX_r GY_r, GX_d^2, GY_d^2, GX_d GY_d]
-// The FP64 form does the same arithmetic in double (exact too, every value being an integer below 2^53); it is the
-// per-frame baseline the integer form is timed against.
-//
-// One wave per strip of 64 columns, moving down WR rows of windows with the running 3- and 9-sums of the last rows
-// in registers; each row of both pictures is staged through LDS with 4 columns of margin either side.
+// Two forms: the integer strip kernel of the batched path, and an FP64 tile kernel (the plain per-frame form, exact
+// too: every value is an integer below 2^53), which also writes the reference's float outputs.
 //
 // persist (src/kernels.cu:430-451): per window of a centre frame, from the source's mid planes of the 6 frames either
 // side, SA = sum at offsets +-2, 4, 6, SB = sum at +-1, 3, 5 (6 * 81 times the reference's A and B), ST = SA + SB
@@ -18,23 +15,6 @@
 #include "energy.hpp"
 
 namespace es {
-
-constexpr int kWsRows = 2;  // rows of windows per wave
-
-template <typename T>
-struct WsPic {
-    // running sums of the last rows, newest last
-    T h3c[3], h3l[3], h3r[3], h9[9], yv[5];
-    T s3c[5], s3l[4], s3r[4];
-};
-
-template <typename T, int N>
-__device__ __forceinline__ void push(T (&a)[N], T v)
-{
-#pragma unroll
-    for (int k = 0; k < N - 1; k++) a[k] = a[k + 1];
-    a[N - 1] = v;
-}
 
 __device__ __forceinline__ int in_sample(const PlaneIn &in, int x, int y)
 {
@@ -51,106 +31,146 @@ __device__ __forceinline__ A reduce_lanes(A v, int width)
     return v;
 }
 
-// T: the arithmetic type (int or double); A: the accumulator type (long long or double).
-template <int WIN, typename T, typename A>
-__global__ __launch_bounds__(256) void window_stats_kernel(const WsPair *pairs)
+// ---------------------------------------------------------------------------------------------------------------
+// FP64 form: the per-frame baseline the integer form is timed against, and the reference-shaped float outputs.
+// One block of 256 threads per tile of 64 columns by one row of windows, taken in bands of 16 rows.  For a band, both
+// pictures' samples of rows -4 .. 19 and columns -4 .. 67 around it (edges repeated) are staged in LDS; the horizontal
+// 3-sums (columns -1 .. 64) and 9-sums (columns 0 .. 63) of every staged row follow, then the vertical 3-sums S3 of
+// rows -1 .. 16.  The box sums are integers; every per-pixel value (F = 9 y - S3, M = 9 S3 - S9, GX and GY the central
+// differences of S3) and every product and sum is then taken in double, which is exact (all integers below 2^53).
+// Thread t takes the column t % 64 at rows t / 64 + 4 k of each band and keeps its 12 sums over the tile's bands; the
+// tile's 64 / WIN windows then add their columns in LDS.  Outputs (any may be null): the 12 sums per window, the
+// reference's 8 figures per window as floats, M as int32 or mid = M / 81 as float on the windows' area.
+// ---------------------------------------------------------------------------------------------------------------
+constexpr int kTileCols = 64, kBand = 16;
+constexpr int kStageRows = kBand + 8, kStageCols = kTileCols + 8;  // rows -4 .. 19, columns -4 .. 67
+constexpr int kH3Cols = kTileCols + 2;                              // columns -1 .. 64
+constexpr int kS3Rows = kBand + 2;                                  // rows -1 .. 16
+
+struct alignas(16) TileLds {
+    int ys[2][kStageRows][kStageCols];
+    int h3[2][kStageRows][kH3Cols];
+    int h9[2][kStageRows][kTileCols];
+    int s3[2][kS3Rows][kH3Cols];
+};
+static_assert(sizeof(TileLds) >= 12 * 256 * sizeof(double), "the reduction reuses the tile's LDS");
+
+// The reference's 8 figures of a window (src/kernels.cu:340-341) from its 12 exact sums.
+__device__ __forceinline__ void figures_of(const double (&z)[12], float (&f)[8])
 {
-    __shared__ int rowbuf[2][4][2][64 + 8];  // double-buffered rows, per wave and picture
-    const WsPair &P = pairs[blockIdx.z];
-    const int lane = threadIdx.x & 63, wave = wave_index();
-    const int w = P.w, h = P.h, ww = w / WIN, wh = h / WIN, W = ww * WIN;
-    const int x0 = (blockIdx.x * 4 + wave) * 64, wy0 = blockIdx.y * kWsRows;
-    if (wy0 >= wh || blockIdx.x * 256 >= W) return;  // uniform over the block
-    const int x = x0 + lane;
-    const bool inside = x < W;
-    const int y0 = wy0 * WIN, y1 = min(wy0 + kWsRows, wh) * WIN;
-    WsPic<T> pic[2];
+    f[0] = __double2float_rn(z[0] / 81.0);
+    f[1] = __double2float_rn(z[1] / 6561.0);
+    f[2] = __double2float_rn(z[2] / 81.0);
+    f[3] = __double2float_rn(z[3] / 6561.0);
+    f[4] = __double2float_rn(z[4] / 81.0);
+    f[5] = __double2float_rn(z[5] / 6561.0);
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) pic[p].h3c[k] = pic[p].h3l[k] = pic[p].h3r[k] = 0;
-#pragma unroll
-        for (int k = 0; k < 9; k++) pic[p].h9[k] = 0;
-#pragma unroll
-        for (int k = 0; k < 5; k++) pic[p].yv[k] = pic[p].s3c[k] = 0;
-#pragma unroll
-        for (int k = 0; k < 4; k++) pic[p].s3l[k] = pic[p].s3r[k] = 0;
+        const double d = z[6 + 3 * p] - z[7 + 3 * p], x = z[8 + 3 * p];
+        f[6 + p] = __double2float_rn(sqrt(__dadd_rn(__dmul_rn(d, d), __dmul_rn(4.0, __dmul_rn(x, x)))) / 81.0);
     }
-    A acc[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) acc[k] = 0;
+}
+
+template <int WIN>
+__global__ __launch_bounds__(256) void window_stats_tile_kernel(const WsPair *pairs)
+{
+    __shared__ TileLds L;
+    const WsPair &P = pairs[blockIdx.z];
+    const int t = threadIdx.x;
+    const int w = P.w, h = P.h, ww = w / WIN, W = ww * WIN;
+    const int x0 = blockIdx.x * kTileCols, wy = blockIdx.y;
+    if (x0 >= W || wy >= h / WIN) return;  // uniform over the block
     const PlaneIn *in[2] = {&P.r, &P.d};
-    int32_t *mid[2] = {P.mid_r, P.mid_d};
-    int buf = 0;
-    for (int r = y0 - 4; r < y1 + 4; r++) {
-        const int yy = min(max(r, 0), h - 1);
+    const int col = t & (kTileCols - 1), X = x0 + col;
+    const bool inside = X < W;
+    double acc[12];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            rowbuf[buf][wave][p][lane + 4] = in_sample(*in[p], min(x, w - 1), yy) << P.up;
-            if (lane < 8) {
-                const int xh = lane < 4 ? x0 - 4 + lane : x0 + 60 + lane;
-                rowbuf[buf][wave][p][lane < 4 ? lane : lane + 64] = in_sample(*in[p], min(max(xh, 0), w - 1), yy) << P.up;
+    for (int k = 0; k < 12; k++) acc[k] = 0.0;
+    for (int band = 0; band < WIN / kBand; band++) {
+        const int yb = wy * WIN + band * kBand;
+        __syncthreads();  // the previous band's readers are done
+        for (int i = t; i < 2 * kStageRows * kStageCols; i += 256) {
+            const int p = i / (kStageRows * kStageCols), r = i / kStageCols % kStageRows, c = i % kStageCols;
+            const int yy = min(max(yb - 4 + r, 0), h - 1), xx = min(max(x0 - 4 + c, 0), w - 1);
+            L.ys[p][r][c] = in_sample(*in[p], xx, yy) << P.up;
+        }
+        __syncthreads();
+        for (int i = t; i < 2 * kStageRows * kH3Cols; i += 256) {
+            const int p = i / (kStageRows * kH3Cols), r = i / kH3Cols % kStageRows, c = i % kH3Cols;
+            const int *y = L.ys[p][r];
+            L.h3[p][r][c] = y[c + 2] + y[c + 3] + y[c + 4];  // column c - 1
+            if (c < kTileCols) {
+                int s9 = 0;
+#pragma unroll
+                for (int j = 0; j < 9; j++) s9 += y[c + j];  // column c
+                L.h9[p][r][c] = s9;
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-            T v[9];
-#pragma unroll
-            for (int d = 0; d < 9; d++) v[d] = (T)rowbuf[buf][wave][p][lane + d];  // columns x - 4 .. x + 4
-            T s9 = 0;
-#pragma unroll
-            for (int d = 0; d < 9; d++) s9 += v[d];
-            WsPic<T> &q = pic[p];
-            push(q.h3l, v[2] + v[3] + v[4]);
-            push(q.h3c, v[3] + v[4] + v[5]);
-            push(q.h3r, v[4] + v[5] + v[6]);
-            push(q.h9, s9);
-            push(q.yv, v[4]);
-            // 3x3 sums of the row before this one (complete now that this row is in)
-            push(q.s3c, q.h3c[0] + q.h3c[1] + q.h3c[2]);
-            push(q.s3l, q.h3l[0] + q.h3l[1] + q.h3l[2]);
-            push(q.s3r, q.h3r[0] + q.h3r[1] + q.h3r[2]);
+        for (int i = t; i < 2 * kS3Rows * kH3Cols; i += 256) {
+            const int p = i / (kS3Rows * kH3Cols), r = i / kH3Cols % kS3Rows, c = i % kH3Cols;
+            L.s3[p][r][c] = L.h3[p][r + 2][c] + L.h3[p][r + 3][c] + L.h3[p][r + 4][c];  // row r - 1
         }
-        buf ^= 1;
-        const int y = r - 4;  // the row whose sums are complete
-        if (y < y0) continue;
-        T F[2], M[2], GX[2], GY[2];
+        __syncthreads();
+        for (int j = t / kTileCols; j < kBand; j += 256 / kTileCols) {
+            const int Y = yb + j;
+            double F[2], M[2], GX[2], GY[2];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            const WsPic<T> &q = pic[p];
-            T s9 = 0;
+            for (int p = 0; p < 2; p++) {
+                const int s3 = L.s3[p][j + 1][col + 1];
+                int s9 = 0;
 #pragma unroll
-            for (int k = 0; k < 9; k++) s9 += q.h9[k];
-            const T s3 = q.s3c[1];  // rows: s3c[0] = y - 1, s3c[1] = y, s3c[2] = y + 1
-            F[p] = (T)9 * q.yv[0] - s3;
-            M[p] = (T)9 * s3 - s9;
-            GX[p] = (x > 0 && x < w - 1) ? q.s3r[0] - q.s3l[0] : (T)0;
-            GY[p] = (y > 0 && y < h - 1) ? q.s3c[2] - q.s3c[0] : (T)0;
-            if (mid[p] && inside) mid[p][(size_t)y * w + x] = (int32_t)M[p];
-        }
-        acc[0] += (A)F[0] * (A)F[0];
-        acc[1] += (A)M[0] * (A)M[0];
-        acc[2] += (A)F[1] * (A)F[1];
-        acc[3] += (A)M[1] * (A)M[1];
-        acc[4] += (A)F[0] * (A)F[1];
-        acc[5] += (A)M[0] * (A)M[1];
-        acc[6] += (A)GX[0] * (A)GX[0];
-        acc[7] += (A)GY[0] * (A)GY[0];
-        acc[8] += (A)GX[0] * (A)GY[0];
-        acc[9] += (A)GX[1] * (A)GX[1];
-        acc[10] += (A)GY[1] * (A)GY[1];
-        acc[11] += (A)GX[1] * (A)GY[1];
-        if (y % WIN == WIN - 1) {
-#pragma unroll
-            for (int k = 0; k < 12; k++) acc[k] = reduce_lanes(acc[k], WIN);
-            if (inside && (x % WIN) == 0) {
-                long long *o = P.sums + ((size_t)(y / WIN) * ww + x / WIN) * 12;
-#pragma unroll
-                for (int k = 0; k < 12; k++) o[k] = (long long)acc[k];
+                for (int k = 0; k < 9; k++) s9 += L.h9[p][j + k][col];
+                F[p] = 9.0 * (double)L.ys[p][j + 4][col + 4] - (double)s3;
+                M[p] = 9.0 * (double)s3 - (double)s9;
+                GX[p] = (X > 0 && X < w - 1) ? (double)(L.s3[p][j + 1][col + 2] - L.s3[p][j + 1][col]) : 0.0;
+                GY[p] = (Y > 0 && Y < h - 1) ? (double)(L.s3[p][j + 2][col + 1] - L.s3[p][j][col + 1]) : 0.0;
             }
+            if (inside) {
+                const size_t o = (size_t)Y * w + X;
+                if (P.mid_r) P.mid_r[o] = (int32_t)M[0];
+                if (P.mid_d) P.mid_d[o] = (int32_t)M[1];
+                if (P.midf_r) P.midf_r[o] = __double2float_rn(M[0] / 81.0);
+                if (P.midf_d) P.midf_d[o] = __double2float_rn(M[1] / 81.0);
+            }
+            const double v[12] = {F[0] * F[0], M[0] * M[0], F[1] * F[1], M[1] * M[1], F[0] * F[1], M[0] * M[1],
+                                  GX[0] * GX[0], GY[0] * GY[0], GX[0] * GY[0], GX[1] * GX[1], GY[1] * GY[1], GX[1] * GY[1]};
 #pragma unroll
-            for (int k = 0; k < 12; k++) acc[k] = 0;
+            for (int k = 0; k < 12; k++) acc[k] += v[k];
+        }
+    }
+    // the tile's windows: each column's 4 row groups, then each window's WIN columns (exact: integers in double)
+    __syncthreads();
+    double(*red)[256] = reinterpret_cast<double(*)[256]>(&L);
+#pragma unroll
+    for (int k = 0; k < 12; k++) red[k][t] = inside ? acc[k] : 0.0;
+    __syncthreads();
+    constexpr int kWins = kTileCols / WIN;
+    __shared__ double win_sum[kWins][12];
+    if (t < kWins * 12) {
+        const int wi = t / 12, k = t % 12;
+        double s = 0.0;
+        for (int c = wi * WIN; c < (wi + 1) * WIN; c++) s += red[k][c] + red[k][c + 64] + red[k][c + 128] + red[k][c + 192];
+        win_sum[wi][k] = s;
+    }
+    __syncthreads();
+    if (t < kWins) {
+        const int wx = x0 / WIN + t;
+        if (wx < ww) {
+            const size_t o = (size_t)wy * ww + wx;
+            double z[12];
+#pragma unroll
+            for (int k = 0; k < 12; k++) z[k] = win_sum[t][k];
+            if (P.sums) {
+#pragma unroll
+                for (int k = 0; k < 12; k++) P.sums[o * 12 + k] = (long long)z[k];
+            }
+            if (P.vals) {
+                float f[8];
+                figures_of(z, f);
+#pragma unroll
+                for (int k = 0; k < 8; k++) P.vals[o * 8 + k] = f[k];
+            }
         }
     }
 }
@@ -371,8 +391,8 @@ static void launch_ws(const WsPair *pairs, int npairs, int max_w, int max_h, boo
 {
     const int wcov = max_w / WIN * WIN, wh = max_h / WIN;
     if (fp64) {
-        const dim3 grid((wcov + 255) / 256, (wh + kWsRows - 1) / kWsRows, npairs);
-        hipLaunchKernelGGL((window_stats_kernel<WIN, double, double>), grid, dim3(256), 0, st, pairs);
+        const dim3 grid((wcov + kTileCols - 1) / kTileCols, wh, npairs);
+        hipLaunchKernelGGL(window_stats_tile_kernel<WIN>, grid, dim3(256), 0, st, pairs);
     } else {
         const dim3 grid((wcov + 63) / 64, (wh * WIN + kWsStrip - 1) / kWsStrip, npairs);
         hipLaunchKernelGGL(window_stats_int_kernel<WIN>, grid, dim3(64), 0, st, pairs);
@@ -534,6 +554,34 @@ __global__ __launch_bounds__(256) void lasting_kernel(const LastingJob *jobs)
         J.sa[i0 + k] = sa[k];
         J.sb[i0 + k] = sb[k];
     }
+}
+
+// The reference-shaped lasting texture (src/kernels.cu:435-451) on float mid planes: per 16 x 16 window of the centre
+// frame, A and B the means of the source's mid planes at offsets +-2, 4, 6 and +-1, 3, 5, T = (A + B) / 2, and the
+// window sums [m(c) T, mid_d T, A B, A^2, B^2], in double, written as floats.  One block of 256 threads per window.
+__global__ __launch_bounds__(256) void persist_f32_kernel(const PersistF32Job *jobs)
+{
+    __shared__ double part[5][4];
+    const PersistF32Job &J = jobs[blockIdx.z];
+    const int wx = blockIdx.x, wy = blockIdx.y, ww = J.w / 16;
+    if (wx >= ww || wy >= J.h / 16) return;
+    const size_t k = (size_t)(wy * 16 + (threadIdx.x >> 4)) * J.w + wx * 16 + (threadIdx.x & 15);
+    const double a = ((double)J.ring[0][k] + J.ring[2][k] + J.ring[4][k] + J.ring[8][k] + J.ring[10][k] + J.ring[12][k]) / 6.0;
+    const double b = ((double)J.ring[1][k] + J.ring[3][k] + J.ring[5][k] + J.ring[7][k] + J.ring[9][k] + J.ring[11][k]) / 6.0;
+    const double t = (a + b) / 2.0;
+    double v[5] = {(double)J.ring[6][k] * t, (double)J.mid_d[k] * t, a * b, a * a, b * b};
+    block_sums(v, part);
+    if (threadIdx.x == 0) {
+        float *o = J.out + ((size_t)wy * ww + wx) * 5;
+        for (int i = 0; i < 5; i++) o[i] = __double2float_rn(v[i]);
+    }
+}
+
+void launch_persist_f32(const PersistF32Job *jobs, int njobs, int max_w, int max_h, hipStream_t st)
+{
+    if (njobs <= 0 || max_w < 16 || max_h < 16) return;
+    hipLaunchKernelGGL(persist_f32_kernel, dim3(max_w / 16, max_h / 16, njobs), dim3(256), 0, st, jobs);
+    hip_check(hipGetLastError(), "lasting texture");
 }
 
 template <int WIN>

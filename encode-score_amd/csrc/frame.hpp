@@ -30,6 +30,7 @@ void launch_frame_planes(const es_frame &f, int x0, int y0, int lshift, uint16_t
 void launch_plane_sse(const uint16_t *a, const uint16_t *b, const float *fa, const float *fb, size_t n,
                       unsigned long long *sse, hipStream_t st);
 void launch_color_sums(const uint16_t *u, const uint16_t *v, size_t n, int up, unsigned long long *sums, hipStream_t st);
+void launch_color_to_double(const unsigned long long *exact, double *sums, hipStream_t st);
 // jobs: device array; max_h: the tallest job's luma height
 void launch_frame_sums(const FrameSumJob *jobs, int njobs, int max_h, hipStream_t st);
 

@@ -212,6 +212,23 @@ void launch_color_sums(const uint16_t *u, const uint16_t *v, size_t n, int up, u
     hip_check(hipGetLastError(), "color sums");
 }
 
+// The reference's color_sums output (src/kernels.cu:470-484) from the exact sums: adds sum u, sum v and the strength
+// sum (units of 2^-23) to three doubles, as the reference's kernel adds into its caller's cleared sums.
+__global__ void color_to_double_kernel(const unsigned long long *exact, double *sums)
+{
+    if (threadIdx.x == 0) {
+        sums[0] += (double)exact[0];
+        sums[1] += (double)exact[1];
+        sums[2] += ldexp((double)exact[2], -23);
+    }
+}
+
+void launch_color_to_double(const unsigned long long *exact, double *sums, hipStream_t st)
+{
+    hipLaunchKernelGGL(color_to_double_kernel, dim3(1), dim3(64), 0, st, exact, sums);
+    hip_check(hipGetLastError(), "color sums");
+}
+
 // ---------------------------------------------------------------------------------------------------------------
 // frame_sums: per block kFrameSumRows luma rows and half as many chroma rows of one job.  A lane takes kSumUnroll
 // columns 256 apart (clamped to the row, the surplus ones counted as zero) and walks down the block's rows two at a

@@ -1,5 +1,5 @@
-// The run's reports in the reference's formats: the JSON file (src/main.cpp:693-777) and the text summary
-// (src/main.cpp:695-727), from the scorer's per-segment figures and per-frame lists.
+// The run's JSON report in the reference's format (src/main.cpp:693-777), from the scorer's per-segment figures and
+// per-frame lists.
 #pragma once
 #include "encscore.h"
 
@@ -15,6 +15,5 @@ struct ReportEncode {
 };
 
 void write_json(FILE *out, const es_run_info &info, long frames, const std::vector<ReportEncode> &encs);
-void write_summary(FILE *out, const es_run_info &info, double seconds, long frames, const std::vector<ReportEncode> &encs);
 
 }  // namespace es

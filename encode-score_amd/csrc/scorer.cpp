@@ -117,23 +117,23 @@ public:
     int window_base, up;
     long frames = 0;  // frame steps submitted
     int chroma = -1;  // whether the run's frames carry chroma planes (-1 before the first frame)
-    std::vector<std::unique_ptr<CambiCacheFile>> caches;  // per crop, when the run uses source-CAMBI caches
+    std::vector<std::unique_ptr<SourceCambiStore>> caches;  // per crop, when the run uses source-CAMBI caches
     long cache_hits = 0;
 
     void use_cache(const std::string &path, const std::string &dir)
     {
         caches.clear();
         for (const Crop &cr : crops)
-            caches.push_back(std::make_unique<CambiCacheFile>(path, cr.cx, cr.cy, cr.w, cr.h, cfg.depth, dir));
+            caches.push_back(std::make_unique<SourceCambiStore>(path, cr.cx, cr.cy, cr.w, cr.h, cfg.depth, dir));
     }
     // The cached CAMBI of crop c's source picture in the frame of timestamp pts.
     bool cached_source(int c, int64_t pts, double *score) const
     {
-        return !caches.empty() && caches[c]->find(pts, score);
+        return !caches.empty() && caches[c]->lookup(pts, score);
     }
     void keep_source(int c, int64_t pts, double score)
     {
-        if (!caches.empty()) caches[c]->add(pts, score);
+        if (!caches.empty()) caches[c]->keep(pts, score);
     }
 
     // The frames of a run as plain luma planes of the config's formats (es_scorer_submit).
@@ -1282,20 +1282,6 @@ int es_scorer_write_json(es_scorer *s, const es_run_info *info, const char *path
         require(f != nullptr, std::string("cannot write ") + path);
         write_json(f, *info, s->impl->frames, r);
         require(std::fclose(f) == 0, std::string("cannot write ") + path);
-    });
-}
-
-int es_scorer_write_summary(es_scorer *s, const es_run_info *info, double seconds, const char *path)
-{
-    return guard([&] {
-        require(s != nullptr, "null scorer");
-        check_info(info);
-        const std::vector<ReportEncode> r = report_of(*s->impl);
-        FILE *f = path ? std::fopen(path, "w") : stdout;
-        require(f != nullptr, std::string("cannot write ") + (path ? path : ""));
-        write_summary(f, *info, seconds, s->impl->frames, r);
-        if (path) require(std::fclose(f) == 0, std::string("cannot write ") + path);
-        else std::fflush(stdout);
     });
 }
 

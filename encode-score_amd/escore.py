@@ -127,6 +127,8 @@ for _n, _a in [
     ("es_window_stats", [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i, _vp]),
     ("es_window_values", [_vp, _i, _vp]),
     ("es_persist_stats", [_vp, _sz, _i, _vp, _i, _i, _i, _vp, _vp]),
+    ("es_window_stats_f32", [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    ("es_persist_stats_f32", [_vp, _sz, _i, _vp, _i, _i, _vp, _vp]),
     ("es_select_rank_u32", [_vp, _sz, _sz, C.POINTER(C.c_uint32), _vp]),
     ("es_select_rank_u64", [_vp, _sz, _sz, C.POINTER(C.c_uint64), _vp]),
     ("es_hist_quantiles", [_vp, _i, _vp, _i, _vp, _vp]),
@@ -148,7 +150,6 @@ for _n, _a in [
     ("es_scorer_psnr", [_vp, _i, C.c_long, _vp]),
     ("es_scorer_set_vmaf", [_vp, _i, C.c_long, _vp]),
     ("es_scorer_write_json", [_vp, C.POINTER(RunInfo), C.c_char_p]),
-    ("es_scorer_write_summary", [_vp, C.POINTER(RunInfo), C.c_double, C.c_char_p]),
     ("es_cambi_cache_open", [C.c_char_p, _i, _i, _i, _i, _i, C.c_char_p, C.POINTER(_vp)]),
     ("es_cambi_cache_find", [_vp, C.c_int64, _dp, C.POINTER(_i)]),
     ("es_cambi_cache_add", [_vp, C.c_int64, C.c_double]), ("es_cambi_cache_close", [_vp]),
@@ -340,6 +341,16 @@ def window_values(sums):
     return out
 
 
+def window_stats_f32(yr, yd, up, w, h, mid_r, mid_d, out, stream=None):
+    """The reference's window_stats outputs: float mid planes and 8 floats per 16 x 16 window."""
+    check(lib.es_window_stats_f32(_ptr(yr), _ptr(yd), up, w, h, _ptr(mid_r), _ptr(mid_d), _ptr(out), _ptr(stream)))
+
+
+def persist_stats_f32(ring, plane, centre, mid_d, w, h, out, stream=None):
+    """The reference's persist_stats on 13 float mid planes: 5 floats per 16 x 16 window."""
+    check(lib.es_persist_stats_f32(_ptr(ring), plane, centre, _ptr(mid_d), w, h, _ptr(out), _ptr(stream)))
+
+
 def persist_stats(ring, plane, centre, mid_d, w, h, win, sums, stream=None):
     check(lib.es_persist_stats(_ptr(ring), plane, centre, _ptr(mid_d), w, h, win, _ptr(sums), _ptr(stream)))
 
@@ -425,10 +436,6 @@ class Scorer:
         """info: dict of source, paths, model, start, offset, fps, segment_seconds, width, height."""
         ri = self._info(info)
         check(lib.es_scorer_write_json(self.ptr, C.byref(ri), str(path).encode()))
-
-    def write_summary(self, info, seconds, path=None):
-        ri = self._info(info)
-        check(lib.es_scorer_write_summary(self.ptr, C.byref(ri), seconds, None if path is None else str(path).encode()))
 
     def submit(self, src_ptrs, enc_ptrs, stream=None):
         """src_ptrs: device pointers of n source frames; enc_ptrs: n x n_encodes device pointers."""

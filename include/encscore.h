@@ -59,6 +59,18 @@ int es_memcpy_dtoh(void *dst, const void *src, size_t bytes, es_stream s);
 int es_memcpy_dtod(void *dst, const void *src, size_t bytes, es_stream s);
 int es_memset(void *dst, int value, size_t bytes, es_stream s);
 
+/* The by-name kernels (replaces Kernels::get / Kernels::launch, src/gpu.hpp:15-23, src/gpu.cpp:8-19, and the launches
+ * by name of src/main.cpp:229-234, :513-566).  es_kernel_get knows the measure path's kernels: "frame_planes",
+ * "window_stats", "persist_stats", "plane_sse" and "color_sums" (ES_EINVAL for any other name).  es_kernel_launch
+ * queues one on `s` with `args` the reference kernel's argument array (one pointer per argument, in the order of
+ * src/kernels.cu's signatures; frame_planes takes an es_frame where the reference's takes its struct Frame, which has
+ * the same fields); w x h is the picture the launch covers, whose height persist_stats takes from it.  The outputs
+ * are the reference's: window_stats and persist_stats as es_window_stats_f32 and es_persist_stats_f32, plane_sse adds
+ * to *sse, color_sums adds sum u, sum v and the strength sum to its three doubles. */
+typedef struct es_kernel_s *es_kernel;
+int es_kernel_get(const char *name, es_kernel *out);
+int es_kernel_launch(es_kernel k, unsigned w, unsigned h, void **args, es_stream s);
+
 /* ---- CAMBI (replaces class Cambi, src/cambi.hpp:9-35, src/cambi.cpp:31-184) ---- */
 
 /* The parameters Cambi::Cambi derives from the picture size (src/cambi.cpp:39-58); window_base is 65 in the
@@ -117,6 +129,13 @@ int es_cambi_cvalues(es_cambi *c, int scale, float *host_out);
  * the area the windows cover.  fp64 != 0 runs the double-precision form of the same arithmetic. */
 int es_window_stats(const uint16_t *yr, const uint16_t *yd, int up, int w, int h, int win, int32_t *mid_r,
                     int32_t *mid_d, int64_t *sums, int fp64, es_stream s);
+/* window_stats(yr, yd, up, w, h, mid_r, mid_d, out) with the reference's outputs (src/kernels.cu:331-347, launched at
+ * src/main.cpp:539-542): 16 x 16 windows, mid_r / mid_d the float mid planes (box3 - box9 of each picture on the area
+ * whole windows cover; either may be NULL), out 8 floats per window [fine_r^2, mid_r^2, fine_d^2, mid_d^2,
+ * fine_r fine_d, mid_r mid_d, oriented_r, oriented_d] (sums over the window).  Each figure is the exact value rounded
+ * once to float (the reference sums in float, so its low bits differ by that rounding). */
+int es_window_stats_f32(const uint16_t *yr, const uint16_t *yd, int up, int w, int h, float *mid_r, float *mid_d,
+                        float *out, es_stream s);
 /* es_window_values: the reference's 8 figures per window (kernels.cu:340-341) from the 12 sums, on the host. */
 int es_window_values(const int64_t *sums, int windows, double *values);
 /* persist_stats(ring, plane, centre, mid_d, w, out): `ring` holds 13 int32 mid planes `plane` elements apart, the
@@ -124,6 +143,13 @@ int es_window_values(const int64_t *sums, int windows, double *values);
  * [M_r(c) ST, M_d(c) ST, SA SB, SA^2, SB^2] with SA/SB the sums at even/odd offsets and ST = SA + SB. */
 int es_persist_stats(const int32_t *ring, size_t plane, int centre, const int32_t *mid_d, int w, int h, int win,
                      int64_t *sums, es_stream s);
+/* persist_stats(ring, plane, centre, mid_d, w, out) with the reference's inputs and outputs (src/kernels.cu:430-451,
+ * launched at src/main.cpp:558-566 over (w / 16) x (h / 16) windows): `ring` holds 13 float mid planes `plane`
+ * elements apart, the centre frame's at slot `centre`; per 16 x 16 window out receives 5 floats [sum m(c) T,
+ * sum mid_d T, sum A B, sum A^2, sum B^2] with A, B the means at offsets +-2, 4, 6 and +-1, 3, 5 and T = (A + B) / 2,
+ * evaluated in double and rounded to float. */
+int es_persist_stats_f32(const float *ring, size_t plane, int centre, const float *mid_d, int w, int h, float *out,
+                         es_stream s);
 
 /* ---- selects (replace the radix select of src/kernels.cu:71-84 + src/cambi.cpp:144-157, quantile of
  *      src/main.cpp:279-289 and Histogram::quantile of src/stats.hpp:21-30) ---- */
@@ -317,9 +343,6 @@ typedef struct {
  * segments and the per-frame lists; the same field names and digits, null for a figure that could not be measured)
  * written to `path`. */
 int es_scorer_write_json(es_scorer *s, const es_run_info *info, const char *path);
-/* The run's text summary as the reference prints it (src/main.cpp:695-727): the whole run and the 5 worst segments of
- * each encode, after a header with the wall time `seconds`; to `path`, or to standard output when path is NULL. */
-int es_scorer_write_summary(es_scorer *s, const es_run_info *info, double seconds, const char *path);
 /* Batched form with profile set: the device milliseconds of each stage summed over the synced rounds, for
  * ES_STAGES stages in the order ES_STAGE_NAMES lists, and the number of rounds.  Each stage is timed on its own
  * stream, so with overlapping chains the stages' times add up to more than the round's. */

@@ -50,6 +50,48 @@ def test_window_stats_8bit_up2(gpu, orc):
     assert np.array_equal(sums.host(), ref_sums)
 
 
+@pytest.mark.parametrize("size", [(256, 144), (333, 211), (1920, 1080)])
+def test_window_stats_f32_reference_outputs(gpu, orc, size):
+    """es_window_stats_f32 writes what the reference's window_stats writes (src/kernels.cu:340-347): float mid planes
+    and 8 floats per 16 x 16 window, each the exact value rounded once to float."""
+    w, h = size
+    print(f"case window_stats_f32 {w}x{h}", flush=True)
+    src, enc = pair(gpu, w, h, 70 + w)
+    _, ref_stats, ref_mr, ref_md = orc.window_stats(src, enc, 0, 16)
+    H, W = (h // 16) * 16, (w // 16) * 16
+    dr, dd = gpu.DeviceArray.from_host(src), gpu.DeviceArray.from_host(enc)
+    mr, md = gpu.DeviceArray((h, w), np.float32), gpu.DeviceArray((h, w), np.float32)
+    mr.zero()
+    md.zero()
+    out = gpu.DeviceArray((ref_stats.shape[0], 8), np.float32)
+    gpu.window_stats_f32(dr, dd, 0, w, h, mr, md, out)
+    assert np.array_equal(out.host().view(np.uint32), ref_stats.astype(np.float32).view(np.uint32))
+    for got, ref in ((mr.host(), ref_mr), (md.host(), ref_md)):
+        want = (ref[:H, :W].astype(np.float64) / 81.0).astype(np.float32)
+        assert np.array_equal(got[:H, :W].view(np.uint32), want.view(np.uint32))
+        assert not got[H:, :].any() and not got[:, W:].any()  # only the windows' area is written
+
+
+def test_persist_f32_reference_outputs(gpu, orc):
+    """es_persist_stats_f32 on float mid planes (mid = M / 81) against the oracle's exact sums on the M planes, scaled
+    to the reference's units (src/kernels.cu:441-450): within float rounding."""
+    rng = np.random.default_rng(11)
+    h, w = 160, 240
+    ring = [rng.integers(-90000, 90000, (h, w)).astype(np.int32) for _ in range(13)]
+    mid_d = rng.integers(-90000, 90000, (h, w)).astype(np.int32)
+    ref = orc.persist(ring, mid_d, 16).astype(np.float64) / np.array([78732.0, 78732.0, 236196.0, 236196.0, 236196.0])
+    centre = 9
+    dev_ring = np.empty((13, h, w), np.float32)
+    for o in range(-6, 7):
+        dev_ring[(centre + o + 13) % 13] = ring[o + 6] / 81.0
+    out = gpu.DeviceArray((ref.shape[0], 5), np.float32)
+    gpu.persist_stats_f32(gpu.DeviceArray.from_host(dev_ring), h * w, centre,
+                          gpu.DeviceArray.from_host((mid_d / 81.0).astype(np.float32)), w, h, out)
+    got = out.host().astype(np.float64)
+    scale = np.abs(ref).max(axis=0)
+    assert np.all(np.abs(got - ref) <= 2e-6 * scale + 1e-6 * np.abs(ref))
+
+
 @pytest.mark.parametrize("win", [16, 64])
 def test_persist_exact(gpu, orc, win):
     rng = np.random.default_rng(9)

@@ -63,11 +63,12 @@ def test_synth_is_deterministic_and_in_range(es):
     assert 64 <= x.min() and x.max() <= 940
 
 
-def test_dropin_headers_compile():
+@pytest.mark.parametrize("name", ["dropin_check.cpp", "dropin_main.cpp"])
+def test_dropin_headers_compile(name):
     """The C++ drop-in headers (gpu.hpp / cambi.hpp / cache.hpp over the C ABI) compile with the host compiler alone."""
     import subprocess
-    src = os.path.join(ROOT, "encode-score_amd", "host", "dropin_check.cpp")
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+    src = os.path.join(ROOT, "encode-score_amd", "host", name)
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         "-I", os.path.join(ROOT, "encode-score_amd", "host"), src], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
 
