@@ -420,8 +420,11 @@ void launch_collect(const CollectJob *jobs, int nframes, int nenc, const Collect
 }
 
 // Histogram::quantile for nhist histograms of kHistBins bins and nq quantiles each: out[h * nq + j], NaN when empty.
-__global__ __launch_bounds__(256) void hist_quantile_kernel(const unsigned long long *hists, const double *qs, int nq,
-                                                            double *out, unsigned long long *counts)
+// Q is the quantiles' source: a device array, or a QuantileList passed by value.  Every call writes every entry of out
+// and (when given) of counts.
+template <class Q>
+__global__ __launch_bounds__(256) void hist_quantile_kernel(const unsigned long long *hists, Q qs, int nq, double *out,
+                                                            unsigned long long *counts)
 {
     constexpr int kChunk = (kHistBins + 255) / 256;
     __shared__ unsigned long long csum[256];
@@ -470,7 +473,15 @@ void launch_hist_quantiles(const unsigned long long *hists, int nhist, const dou
                            unsigned long long *counts, hipStream_t st)
 {
     if (nhist <= 0) return;
-    hipLaunchKernelGGL(hist_quantile_kernel, dim3(nhist), dim3(256), 0, st, hists, qs, nq, out, counts);
+    hipLaunchKernelGGL(hist_quantile_kernel<const double *>, dim3(nhist), dim3(256), 0, st, hists, qs, nq, out, counts);
+    hip_check(hipGetLastError(), "histogram quantiles");
+}
+
+void launch_hist_quantiles_list(const unsigned long long *hists, int nhist, const QuantileList &qs, int nq, double *out,
+                                unsigned long long *counts, hipStream_t st)
+{
+    if (nhist <= 0) return;
+    hipLaunchKernelGGL(hist_quantile_kernel<QuantileList>, dim3(nhist), dim3(256), 0, st, hists, qs, nq, out, counts);
     hip_check(hipGetLastError(), "histogram quantiles");
 }
 

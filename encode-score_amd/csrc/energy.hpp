@@ -95,6 +95,14 @@ struct CollectWork {
 void launch_collect(const CollectJob *jobs, int nframes, int nenc, const CollectWork &w, hipStream_t st);
 void launch_hist_quantiles(const unsigned long long *hists, int nhist, const double *qs, int nq, double *out,
                            unsigned long long *counts, hipStream_t st);
+// The quantiles of a launch carried in its arguments.
+constexpr int kMaxListQuantiles = 16;
+struct QuantileList {
+    double q[kMaxListQuantiles];
+    __host__ __device__ double operator[](int j) const { return q[j]; }
+};
+void launch_hist_quantiles_list(const unsigned long long *hists, int nhist, const QuantileList &qs, int nq, double *out,
+                                unsigned long long *counts, hipStream_t st);
 void launch_select_u64(const unsigned long long *keys, long long n, long long rank, unsigned long long *out,
                        hipStream_t st);
 void launch_copy(void *dst, const void *src, size_t bytes, hipStream_t st);

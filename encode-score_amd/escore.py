@@ -131,7 +131,7 @@ for _n, _a in [
     ("es_persist_stats_f32", [_vp, _sz, _i, _vp, _i, _i, _vp, _vp]),
     ("es_select_rank_u32", [_vp, _sz, _sz, C.POINTER(C.c_uint32), _vp]),
     ("es_select_rank_u64", [_vp, _sz, _sz, C.POINTER(C.c_uint64), _vp]),
-    ("es_hist_quantiles", [_vp, _i, _vp, _i, _vp, _vp]),
+    ("es_hist_quantiles", [_vp, _i, _vp, _i, _vp, _vp]), ("es_hist_quantiles_noalloc", [_vp, _i, _vp, _i, _vp, _vp, _vp]),
     ("es_scorer_create", [C.POINTER(ScorerConfig), C.POINTER(_vp)]), ("es_scorer_destroy", [_vp]),
     ("es_scorer_autotune", [C.POINTER(ScorerConfig), _i, _vp, _vp, C.POINTER(TuneResult)]),
     ("es_scorer_submit", [_vp, _i, _vp, _vp, _vp]), ("es_scorer_sync", [_vp]),
@@ -363,6 +363,16 @@ def select_rank(keys, n, rank, stream=None):
         out = C.c_uint64()
         check(lib.es_select_rank_u64(keys.ptr, n, rank, C.byref(out), _ptr(stream)))
     return out.value
+
+
+def hist_quantiles_noalloc(hists, nhist, qs, stream=None):
+    """hist_quantiles through es_hist_quantiles_noalloc (results staged in the library's static device array)."""
+    qs = np.ascontiguousarray(qs, np.float64)
+    out = np.empty((nhist, qs.size), np.float64)
+    counts = np.empty(nhist, np.uint64)
+    check(lib.es_hist_quantiles_noalloc(_ptr(hists), nhist, qs.ctypes.data, qs.size, out.ctypes.data,
+                                        counts.ctypes.data, _ptr(stream)))
+    return out, counts
 
 
 def hist_quantiles(hists, nhist, qs):

@@ -160,6 +160,15 @@ int es_select_rank_u64(const uint64_t *keys, size_t n, size_t rank, uint64_t *ou
 /* Histogram::quantile of nhist device histograms of 20001 uint64 bins each, for nq quantiles: host out[h * nq + j]
  * (NaN for an empty histogram) and host counts[h] (may be NULL). */
 int es_hist_quantiles(const uint64_t *hists, int nhist, const double *qs, int nq, double *out, uint64_t *counts);
+/* es_hist_quantiles without allocating: no counterpart in the reference; for callers that cannot allocate device or
+ * pinned memory (code under graph capture, tight serving loops, harnesses that forbid it).  The same results, queued
+ * on `s` and waited for; the results are staged in a static device array of the library (one per GPU, about 544 KiB,
+ * held until the process ends), so at most ES_NOALLOC_MAX_HISTS histograms and ES_NOALLOC_MAX_QUANTILES quantiles per
+ * call (ES_EINVAL beyond), and calls that overlap on one GPU race on it. */
+#define ES_NOALLOC_MAX_HISTS 4096
+#define ES_NOALLOC_MAX_QUANTILES 16
+int es_hist_quantiles_noalloc(const uint64_t *hists, int nhist, const double *qs, int nq, double *out, uint64_t *counts,
+                              es_stream s);
 
 /* ---- frame ingest and the PSNR and colour reductions (src/kernels.cu:280-310, :453-484; src/main.cpp:128-153,
  *      :273-277, :496-505, :543-556) ---- */
