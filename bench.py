@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--cv-tile-blocks", type=int, default=0, help="c-value kernel as one block per tile (with --tune 0)")
     ap.add_argument("--cv-mfma", type=int, default=0,
                     help="c-value window counts on the matrix cores (with --tune 0)")
+    ap.add_argument("--cv-queues", type=int, default=0,
+                    help="c-value tile queues: one shared queue (0) or one per XCD (1) (with --tune 0)")
     ap.add_argument("--early-energies", type=int, default=0,
                     help="start the energy chain with the CAMBI pyramid rather than after it (with --tune 0)")
     ap.add_argument("--plain-steps", type=int, default=48, help="frame steps of the plain form timed")
@@ -264,7 +266,7 @@ def main():
     # the autotuner, on the pool's first frames, outside the timed region; if it cannot run, the defaults
     tuned = None
     batch, tile_h, cv_bmax, cv_waves, cv_tb = args.batch, args.tile_h, args.cv_bmax, args.cv_waves, bool(args.cv_tile_blocks)
-    early, mfma = bool(args.early_energies), bool(args.cv_mfma)
+    early, mfma, queues = bool(args.early_energies), bool(args.cv_mfma), args.cv_queues
     if args.tune:
         tune_descs = [es.EncodeDesc(d.w, d.h, pool, d.w, 2, 0) for d, _ in encs]
         s_t, e_t = ptrs(0, TUNE_FRAMES)
@@ -274,6 +276,7 @@ def main():
                                 serial_stages=bool(args.serial))
             batch, tile_h, cv_bmax, cv_waves = tuned["batch"], tuned["tile_h"], tuned["cv_bmax"], tuned["cv_waves"]
             cv_tb, early, mfma = tuned["cv_tile_blocks"], tuned["early_energies"], tuned["cv_mfma"]
+            queues = tuned.get("cv_queues", 0)
         except es.EsError as exc:
             tuned = {"fallback": f"the defaults and the command line's settings: the autotuner failed ({exc})"}
         tuned["seconds"] = round(time.perf_counter() - t0, 2)
@@ -290,7 +293,7 @@ def main():
     def batched(**kw):
         return es.Scorer(es.ES_BATCHED, W, H, kw.pop("descs", descs), batch=batch, tile_h=tile_h, use_graph=True,
                          src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves, cv_tile_blocks=cv_tb,
-                         early_energies=early, cv_mfma=mfma, **kw)
+                         early_energies=early, cv_mfma=mfma, cv_queues=queues, **kw)
 
     bat = batched(profile=True, serial_stages=bool(args.serial))
     submit(bat, 0, warm)
@@ -415,6 +418,8 @@ def main():
                    "frame_steps_per_step": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32,
                    "cv_waves": "one block per tile" if cv_tb else (cv_waves or 16),
                    "cv_method": "matrix cores" if mfma else "prefix scans",
+                   "cv_queues": "one block per tile" if cv_tb else ("one per XCD, an eighth of every scale each"
+                                                                     if queues > 0 else "one shared queue"),
                    "chroma": bool(args.chroma), "stages": "serial" if args.serial else
                    ("energies overlapped from the pyramid" if early else "energies overlapped from the c-values"),
                    "form": "batched, integer energies, graph replay"},

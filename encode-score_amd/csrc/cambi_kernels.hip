@@ -986,15 +986,47 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
 template <typename Sh> constexpr int kCvWaves = 3;
 template <> constexpr int kCvWaves<CvSharedM<1>> = 4;
 
+// The tile of claim v on XCD queue k, or -1 when the queue is empty: queue k holds the k-th eighth of every scale's
+// tiles (tile_first lists the slots by scale, coarsest first, np pictures each), coarsest scale first.
+__device__ int queue_tile(const int *tile_first, int np, unsigned k, unsigned v)
+{
+    for (int j = 0; j < kScales; j++) {
+        const unsigned a = (unsigned)tile_first[j * np], n = (unsigned)tile_first[(j + 1) * np] - a;
+        const unsigned lo = (unsigned)((unsigned long long)n * k / kXcds), hi = (unsigned)((unsigned long long)n * (k + 1) / kXcds);
+        if (v < hi - lo) return (int)(a + lo + v);
+        v -= hi - lo;
+    }
+    return -1;
+}
+
 // Sh: CvShared<BMAX> (prefix scans on the vector units) or CvSharedM<NB> (window counts on the matrix cores).
+// Persistent waves take tiles from queues.  With xcd_queues there is one queue per XCD (counter[k]): the blocks
+// launched to XCD k (blockIdx.x % 8) take the k-th eighth of every scale's tiles, coarsest scale first, so that
+// neighbouring tiles, which share halo rows and columns, run together on one XCD and meet in its L2, and every XCD
+// has the same share of every scale's work.  A wave whose queue is empty goes on to the next queues and stops when all
+// eight are empty; each queue's claims rise, so an empty queue stays empty, and every tile is in exactly one queue.
+// Without xcd_queues every wave takes tiles in order from counter[0].
 template <typename Sh>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCvWaves<Sh>))) void cambi_cvalues_kernel(
-    const CambiPic *pics, const int *tile_first, int nslots, int tile_h, int persistent, unsigned int *counter)
+    const CambiPic *pics, const int *tile_first, int nslots, int tile_h, int persistent, unsigned int *counter,
+    int xcd_queues)
 {
     __shared__ Sh sh;
-    const int total = tile_first[nslots];
+    const int total = tile_first[nslots], np = nslots / kScales;
+    const unsigned home = blockIdx.x % kXcds;
+    unsigned q = 0;  // queues left behind as empty
     for (int t = persistent ? 0 : (int)blockIdx.x;;) {
-        if (persistent) {
+        if (persistent && xcd_queues) {
+            t = -1;
+            for (; q < kXcds && t < 0; ) {
+                const unsigned k = (home + q) % kXcds;
+                unsigned v = 0;
+                if (threadIdx.x == 0) v = atomicAdd(counter + k, 1u);
+                t = queue_tile(tile_first, np, k, __builtin_amdgcn_readfirstlane(v));
+                if (t < 0) q++;
+            }
+            if (t < 0) break;
+        } else if (persistent) {
             unsigned v = 0;
             if (threadIdx.x == 0) v = atomicAdd(counter, 1u);
             t = __builtin_amdgcn_readfirstlane((int)v);
@@ -1008,13 +1040,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCvWaves<Sh>
 }
 
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma)
+                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma,
+                          int xcd_queues)
 {
     if (!persistent && total_tiles <= 0) return;
     const int grid = persistent ? blocks : total_tiles;
     auto k = mfma ? (bmax == 64 ? cambi_cvalues_kernel<CvSharedM<2>> : cambi_cvalues_kernel<CvSharedM<1>>)
                   : (bmax == 64 ? cambi_cvalues_kernel<CvShared<64>> : cambi_cvalues_kernel<CvShared<32>>);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h, persistent, work_counter);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h, persistent, work_counter,
+                       xcd_queues);
     hip_check(hipGetLastError(), "cambi c-values");
 }
 

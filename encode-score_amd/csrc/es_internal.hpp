@@ -16,6 +16,7 @@ constexpr uint16_t kNone = 0xFFFF;
 constexpr int kCvTileW = 64;     // c-value tiles are one wave wide
 constexpr int kMaxPad = 32;      // one wave covers the tile and both margins as 2 columns per lane: window <= 65
 constexpr int kMaxBand = 1024;   // band values, 10-bit codes
+constexpr int kXcds = 8;          // XCDs of an MI355X: a launch's workgroups go to them in turn
 constexpr int kHistBins = 20001; // per-window histograms: 0.05-point bins over [-500, 500]
 constexpr int kRing = 13, kRadius = 6;
 constexpr int kMaxTileH = 4096;  // c-value tile heights: a lane's exact double sum of a tile's c-values needs < 2^20
@@ -103,7 +104,8 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // when not persistent.
 // mfma: window counts on the matrix cores instead of prefix scans (same results).
 void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma = 0);
+                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma,
+                          int xcd_queues = 0);
 void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks, hipStream_t st);
 void launch_cambi_finish(const CambiPic *pics, TopkJob *jobs, int npics, double *scores, hipStream_t st);
 void launch_clear(unsigned long long *a, long long na, unsigned int *b, long long nb, hipStream_t st);

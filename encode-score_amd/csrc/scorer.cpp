@@ -452,6 +452,7 @@ public:
         cv_bmax = c.cv_bmax ? c.cv_bmax : 32;
         cv_tile_blocks = c.cv_tile_blocks != 0;
         cv_mfma = c.cv_mfma != 0;
+        cv_queues = c.cv_queues > 0;
         const int C = (int)crops.size(), E = (int)enc.size();
         roles = C + E;
         P = B * roles;
@@ -691,6 +692,7 @@ private:
 
     int B, R, P, roles, tile_h, cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
     bool cv_tile_blocks = false, cv_mfma = false;
+    bool cv_queues = false;  // persistent c-value waves take tiles from one queue per XCD (else one shared queue)
     std::vector<long long> max_px;
     std::vector<int> max_strips;
     struct Layout {
@@ -879,7 +881,8 @@ private:
         switch (k) {
         case 0:
             hip_check(hipMemcpyAsync(desc[p].p, stage_buf[p].p, desc_bytes, hipMemcpyHostToDevice, st), "descriptors");
-            launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(), (long long)P * kScales + 1, st);
+            launch_clear(sums.as<unsigned long long>(), (long long)P * kScales, counts.as<unsigned>(),
+                         (long long)P * kScales + kXcds, st);
             break;
         case 1: launch_cambi_base(pics, nullptr, P, max_base_tiles, st); break;
         case 2:
@@ -889,7 +892,8 @@ private:
             // persistent waves taking tiles from a counter, or one block per tile of the round's largest layout (the
             // blocks past the round's tiles exit at once)
             launch_cambi_cvalues(pics, dptr<int>(p, o_first()), P * kScales, (int)seg_off[P], tile_h, cv_tile_blocks ? 0 : 1,
-                                 counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st, cv_mfma ? 1 : 0);
+                                 counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st, cv_mfma ? 1 : 0,
+                                 cv_queues ? 1 : 0);
             break;
         case 4:
             for (int pass = 0; pass < 3; pass++) launch_topk_pass(topk.as<TopkJob>(), P * kScales, pass, kTopkBlocks, st);
@@ -1324,7 +1328,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         base.profile = 0;
         es_tune_result best{base.batch ? base.batch : 16, base.tile_h ? base.tile_h : 32, base.cv_bmax ? base.cv_bmax : 32,
                             base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks, base.early_energies,
-                            base.cv_mfma};
+                            base.cv_mfma, base.cv_queues};
         const int ne = base.n_encodes;
         auto trial = [&](const es_tune_result &t) -> double {
             es_scorer_config c = base;
@@ -1335,6 +1339,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             c.cv_tile_blocks = t.cv_tile_blocks;
             c.early_energies = t.early_energies;
             c.cv_mfma = t.cv_mfma;
+            c.cv_queues = t.cv_queues;
             const int need = 3 * t.batch;
             if (nframes < need) return -1.0;  // too few frames for this batch
             // one warm-up batch (graph capture included), then the same two batches timed kRepeats times, the best
@@ -1374,7 +1379,7 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
                 set(t, v);
                 if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves &&
                     t.cv_tile_blocks == keep.cv_tile_blocks && t.early_energies == keep.early_energies &&
-                    t.cv_mfma == keep.cv_mfma)
+                    t.cv_mfma == keep.cv_mfma && t.cv_queues == keep.cv_queues)
                     continue;
                 const double ms = trial(t);
                 if (ms > 0 && ms < best.ms_per_step) {
@@ -1386,11 +1391,13 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         // window counts on the matrix cores or by prefix scans
         try_set([](es_tune_result &t, int v) { t.cv_mfma = v; }, {0, 1});
         // tile height and LDS layout together: taller tiles see more values, which favours the wider passes
-        try_set([](es_tune_result &t, int v) { t.tile_h = v % 100, t.cv_bmax = v / 100; },
-                {3216, 3224, 3232, 3248, 3264, 6416, 6424, 6432, 6448, 6464});
+        try_set([](es_tune_result &t, int v) { t.tile_h = v % 1000, t.cv_bmax = v / 1000; },
+                {32016, 32024, 32032, 32048, 32064, 32096, 32128, 64016, 64024, 64032, 64048, 64064, 64096, 64128});
         // with the energy chain overlapping, fewer resident c-value waves can leave it room
         // (0: one block per tile, which the hardware interleaves with the overlapping energy chain)
         try_set([](es_tune_result &t, int v) { t.cv_waves = v, t.cv_tile_blocks = v == 0; }, {0, 6, 8, 12, 16, 24});
+        // the persistent waves' queues: one shared queue (0) or one per XCD (1)
+        if (!best.cv_tile_blocks) try_set([](es_tune_result &t, int v) { t.cv_queues = v; }, {0, 1});
         if (!base.serial_stages) try_set([](es_tune_result &t, int v) { t.early_energies = v; }, {0, 1});
         try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32});
         const int trials = out->trials;

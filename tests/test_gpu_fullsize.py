@@ -111,13 +111,15 @@ def test_c2_mosaic_4k(gpu, orc):
     """C2 at 3840x2160: matrix cores at the autotuner's tile height and prefix scans with the wide layout."""
     scorer_vs_oracle(gpu, orc, 2, 14, [0],
                      settings=(dict(cv_mfma=True, tile_h=64, cv_bmax=32, batch=8),
-                               dict(cv_mfma=False, tile_h=32, cv_bmax=64, batch=8, cv_waves=12)))
+                               dict(cv_mfma=False, tile_h=32, cv_bmax=64, batch=8, cv_waves=12),
+                               dict(cv_mfma=True, tile_h=128, cv_bmax=32, batch=8, cv_queues=1)))
 
 
 def test_c3_uneven_crops(gpu, orc):
     """C3: the four real crops (1920x1080, 1920x800, 1920x1040, 1800x1080) and uneven lengths, every encode."""
     scorer_vs_oracle(gpu, orc, 3, 16, [0, 1, 2, 3], per_segment=9,
-                     settings=(dict(cv_mfma=True, tile_h=64, batch=8), dict(cv_mfma=False, batch=16)))
+                     settings=(dict(cv_mfma=True, tile_h=64, batch=8), dict(cv_mfma=False, batch=16),
+                               dict(cv_mfma=True, tile_h=96, batch=8, cv_queues=1, cv_waves=8)))
 
 
 def test_c4_8bit_source_three_encodes(gpu, orc):
