@@ -543,11 +543,12 @@ def main():
                 return 0.0
             return abs(a - b) if not (np.isnan(a) or np.isnan(b)) else float("inf")
 
-        errs = {k: 0.0 for k in ("cambi", "cambi_source", "texture", "banding", "grain", "score")}
+        scalars = ("cambi", "cambi_source", "texture", "banding", "grain", "score")
+        errs = {k: 0.0 for k in scalars}
         lasting_equal, psnr_exact = True, True
         for i in range(len(encs)):
             got, ref = chk.segment(i, -1), orc.accum_summary(runs[i].merged())[0]
-            for k in errs:
+            for k in scalars:
                 errs[k] = max(errs[k], err(got[k], ref[k]))
             lasting_equal &= bool(got["n_lasting"] == ref["n_lasting"])
             if args.chroma:
