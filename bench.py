@@ -79,8 +79,9 @@ STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{layout}>", "cambi_b
                 "frame_sums": "frame_sums_kernel"}
 # rocprofv3 --pmc passes of each workload (FETCH_SIZE, WRITE_SIZE and the SQ counters of the dominant kernel),
 # summarised per dispatch; see each file's "command"
-PMC_FILES = {(1, 1920, 1080): "pmc_1920x1080_ramp.json", (2, 3840, 2160): "pmc_3840x2160_mosaic.json",
-             (5, 3840, 1600): "pmc_3840x1600_mosaic_8enc.json"}
+PMC_FILES = {(1, 1920, 1080): "pmc_1920x1080_ramp.json", (2, 3840, 2160): "pmc_3840x2160_mosaic.json"}
+PMC_MISSING = {5: "no PMC summary: rocprofv3's first counter pass over this workload ended in a host fault while the "
+                  "run generated its frames (profiles/measure_path_3840x1600_mosaic_8enc.md)"}
 
 
 def kernel_name(stage, bmax, mfma):
@@ -89,15 +90,15 @@ def kernel_name(stage, bmax, mfma):
 
 
 def pmc_profile(cfg, w, h, kernel):
-    """(per-dispatch record of `kernel`, frames per launch, file name) from the committed PMC summary of this
-    workload, or (None, None, reason)."""
+    """(per-dispatch record of `kernel`, the profiled run's input bytes per launch, file name) from the committed PMC
+    summary of this workload, or (None, None, reason)."""
     name = PMC_FILES.get((cfg, w, h))
     if name is None:
-        return None, None, "no rocprofv3 PMC summary of this workload"
+        return None, None, PMC_MISSING.get(cfg, "no rocprofv3 PMC summary of this workload")
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
             prof = json.load(f)
-        return prof["kernels"][kernel], prof["frames_per_launch"], f"profiles/{name}"
+        return prof["kernels"][kernel], prof["b_in_per_launch"], f"profiles/{name}"
     except (OSError, KeyError, ValueError):
         return None, None, f"profiles/{name} holds no entry for {kernel}"
 
@@ -363,7 +364,7 @@ def main():
     b_in_launch = b_in_timed / launches
     achieved = b_in_launch / (mean_ms / 1e3) / 1e9
     kname = kernel_name(dominant, cv_bmax or 32, mfma)
-    rec, prof_frames, prof_src = pmc_profile(args.config, W, H, kname)
+    rec, prof_b_in, prof_src = pmc_profile(args.config, W, H, kname)
     roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                 "algorithmic_bytes_per_launch": int(b_in_launch),
@@ -381,8 +382,9 @@ def main():
                 "counters_source": prof_src}
     if rec is not None:
         # HBM bytes per dispatch from the PMC passes (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
-        # calibration), scaled by frame steps per launch; every kernel's traffic is per frame step of the launch
-        scale = (timed_frames / launches) / prof_frames
+        # calibration), scaled by this run's input bytes per launch over the profiled run's (a kernel's traffic goes
+        # with the pictures of its launch)
+        scale = b_in_launch / prof_b_in
         traffic = rec["bytes"] * scale
         roofline["traffic"] = int(traffic)
         roofline["traffic_over_algorithmic"] = round(traffic / b_in_launch, 3)

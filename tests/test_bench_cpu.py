@@ -42,3 +42,16 @@ def test_config3_and_5_keep_the_reference_proportions(es):
     assert [length for _, length in encs] == [1440, 1100, 720, 333]
     _, encs, _, _, _ = bench.config(5, es, 1440)
     assert [length for _, length in encs][0] == 1440 and [length for _, length in encs][-1] == 500
+
+
+@pytest.mark.parametrize("cfg,w,h,stage", [(1, 1920, 1080, "cambi_cvalues"), (2, 3840, 2160, "cambi_cvalues"),
+                                          (2, 3840, 2160, "window_stats"), (2, 3840, 2160, "persist")])
+def test_pmc_summaries_hold_the_profiled_kernels(cfg, w, h, stage):
+    """The committed rocprofv3 summaries the bench's roofline reads: HBM bytes, counters and the profiled run's input
+    bytes per launch for each kernel it may name."""
+    rec, b_in, src = bench.pmc_profile(cfg, w, h, bench.kernel_name(stage, 32, True))
+    assert rec is not None, src
+    assert rec["bytes"] > 0 and b_in > 0 and src.startswith("profiles/")
+    if stage == "cambi_cvalues":
+        d = rec["derived"]
+        assert 0 < d["wait_any_per_wave_cycle"] < 1 and d["lds_bank_conflict_rate"] > 0 and rec["mfma_i8_ops"] > 0
