@@ -472,21 +472,22 @@ __device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, double &sum_c, 
         for (int r = 0; r < t.rows; r++) {
             const int y = t.y0 + r;
             if (r + 1 < t.rows) load(y + 1, nxt);
-            // every rank the row needs, read at once: the samples entering and leaving, the own sample and the
-            // values within kDiffs of it
+            // the ranks of the samples entering and leaving and of the own sample, read at once; those of the values
+            // within kDiffs of it only for a row the pass serves
             const unsigned io = cv_index(t, cur[2]), base = io ? io - kDiffs : 0u;
             const unsigned re0 = sh.rank[cv_index(t, cur[0])], re1 = sh.rank[cv_index(t, cur[1])];
             const unsigned rl0 = sh.rank[cv_index(t, gone0)], rl1 = sh.rank[cv_index(t, gone1)];
-            unsigned rk[2 * kDiffs + 1];
-#pragma unroll
-            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = sh.rank[base + k];
+            const unsigned rown = sh.rank[base + kDiffs];
             bump(rl0, c0, ~0u);
             bump(rl1, c0 + 1, ~0u);
             bump(re0, c0, 1u);
             bump(re1, c0 + 1, 1u);
-            const unsigned own = io ? rk[kDiffs] : (unsigned)kNone;
+            const unsigned own = io ? rown : (unsigned)kNone;
             const bool serve = own != kNone && (int)own >= lo && (int)own < hi;
             if (__builtin_amdgcn_ballot_w64(serve)) {
+                unsigned rk[2 * kDiffs + 1];
+#pragma unroll
+                for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = k == kDiffs ? rown : sh.rank[base + k];
                 // the chunks of 8 numbers some served pixel reads: its own number and those within 4 of it
                 const int rel = (int)own - b0;
                 unsigned need = 1;
@@ -722,17 +723,19 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             const unsigned io = index(cur[2]);
             const unsigned re0 = rows[index(cur[0])], re1 = rows[index(cur[1])];
             const unsigned rl0 = rows[index(gone0)], rl1 = rows[index(gone1)];
-            unsigned rk[2 * kDiffs + 1];  // rows of the own value and the 4 values either side
-#pragma unroll
-            for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = rows[io - kDiffs + k];
+            const unsigned own = rows[io];
             load(y + 1, nxt);  // past the tile's last row it reads unused samples (past the picture, zeros)
             bump(rl0, 0u - inc0);
             bump(rl1, 0u - inc1);
             bump(re0, inc0);
             bump(re1, inc1);
-            const unsigned own = rk[kDiffs];
             const bool serve = own >= lo_rel && own < hi_rel;
             if (__builtin_amdgcn_ballot_w64(serve)) {
+                // rows of the own value and the 4 values either side, read only for a row the pass serves (they are
+                // not needed until after the products below)
+                unsigned rk[2 * kDiffs + 1];
+#pragma unroll
+                for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = k == kDiffs ? own : rows[io - kDiffs + k];
                 // the LDS operations of one wave complete in order: the row's updates are in before these reads
                 asm volatile("" ::: "memory");
                 cv_v4i bop[NBU][NJ + 1];
