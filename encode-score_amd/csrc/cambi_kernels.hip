@@ -371,6 +371,11 @@ struct CvTile {
     int w, h, x0, y0, rows;
     int pad, band_base, band_size, vlt;
     int tvi[kDiffs], weight[kDiffs];
+    // the thresholds of difference d as one unsigned range test on a sample's index i (value less band_base plus 4):
+    // value + 4 <= tvi[d] and value + 4 + d + 1 > vlt exactly when i - oklo[d] <= okspan[d]; an empty range has
+    // oklo = 2^31 and okspan = 0, which no index below 2^31 passes
+    unsigned oklo[kDiffs], okspan[kDiffs];
+    int keep;  // the full c-value plane is kept
     // the full c-value plane (when kept; else an empty buffer) and this tile's segment of the list, as buffers
     __amdgpu_buffer_rsrc_t plane_rs, seg_rs;
 };
@@ -670,7 +675,11 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
     // this lane's start in the count table (B operand: number r32, its 16 margin columns of a block) and in the
     // window table (the products it writes: number r32, output columns 4 g ..)
     const uint8_t *const crow = &sh.cnt[r32][16 * g];
-    uint16_t *const wrow = &sh.win[r32][4 * g];
+    // the window table's words from this lane's first product on, as one LDS address held in a register, so that
+    // every store of the products is that address plus a small immediate offset
+    typedef __attribute__((address_space(3))) uint32_t lds_word;
+    lds_word *wrow = (lds_word *)&sh.win[r32][4 * g];
+    asm volatile("" : "+v"(wrow));
     const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs), ibase = 0x8000u + (unsigned)t.band_base - kDiffs;
     auto index = [&](unsigned v) { return min(v - ibase, nv); };
     const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the row of index i (i from -4 to nv + 4)
@@ -762,23 +771,24 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
                         for (int q = 0; q < 4; q++)
 #pragma unroll
                             for (int e = 0; e < 2; e++)
-                                *(uint32_t *)(wrow + (32 * nb * kMWin + 32 * mb + 8 * q + 2 * e)) =
+                                wrow[(32 * nb * kMWin + 32 * mb + 8 * q + 2 * e) / 2] =
                                     (unsigned)acc[mb][4 * q + 2 * e] | (unsigned)acc[mb][4 * q + 2 * e + 1] << 16;
                 }
                 asm volatile("" ::: "memory");
                 unsigned cnt9[2 * kDiffs + 1];
 #pragma unroll
                 for (int k = 0; k < 2 * kDiffs + 1; k++) cnt9[k] = sh.win[rk[k]][lane];
-                const int value = (int)io + t.band_base;  // on a served pixel: its value + 4, as the thresholds take it
+                // the index of a served pixel, and past every threshold range on the other lanes, so that one range
+                // test per difference takes both the thresholds and the serving
+                const unsigned ios = serve ? io : 0x40000000u;
                 const unsigned p0 = cnt9[kDiffs];
                 float c = 0.0f;
 #pragma unroll
                 for (int d = 0; d < kDiffs; d++) {
-                    // the thresholds (value <= tvi[d], value + d + 1 > vlt) combined without short-circuiting; a
-                    // difference no served pixel of the row passes is skipped by the whole wave (bright rows pass
+                    // a difference no served pixel of the row passes is skipped by the whole wave (bright rows pass
                     // only the larger differences)
-                    const bool ok = (value <= t.tvi[d]) & (value + d + 1 > t.vlt);
-                    if (!__builtin_amdgcn_ballot_w64(serve & ok)) continue;
+                    const bool ok = ios - t.oklo[d] <= t.okspan[d];
+                    if (!__builtin_amdgcn_ballot_w64(ok)) continue;
                     const unsigned pa = cnt9[kDiffs + d + 1], pb = cnt9[kDiffs - d - 1];
                     const unsigned pm = pa > pb ? pa : pb;
                     // w * p0 * pm * RN(1 / (p0 + pm)), the reciprocal as the reference's table holds it; computed on
@@ -791,11 +801,11 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
                     const bool take = ok & (val > c);
                     c = take ? val : c;
                 }
-                c = serve ? c : 0.0f;
-                // the full plane (when kept) and the tile's list, as buffer stores: a lane with nothing to store
-                // stores past the end, where the store is dropped
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.plane_rs, serve ? 4u * (unsigned)x : 0x80000000u,
-                                                      __builtin_amdgcn_readfirstlane(4u * (unsigned)y * (unsigned)t.w), 0);
+                // c stays 0 on a lane that is not served.  The full plane (when kept) and the tile's list, as buffer
+                // stores: a lane with nothing to store stores past the end, where the store is dropped
+                if (t.keep)
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(c), t.plane_rs, serve ? 4u * (unsigned)x : 0x80000000u,
+                                                          __builtin_amdgcn_readfirstlane(4u * (unsigned)y * (unsigned)t.w), 0);
                 sum_c += (double)c;  // exact: see cv_tile
                 const unsigned long long m = __builtin_amdgcn_ballot_w64(c > 0.0f);
                 const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -905,7 +915,12 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     for (int d = 0; d < kDiffs; d++) {
         t.tvi[d] = __builtin_amdgcn_readfirstlane(K.tvi[d]);
         t.weight[d] = __builtin_amdgcn_readfirstlane(K.weight[d]) & 0xFF;  // 1 .. 4: 24-bit products stay exact
+        // i + band_base <= tvi[d] and i + band_base + d + 1 > vlt, with i >= 0
+        const int lo = max(t.vlt - t.band_base - d, 0), hi = t.tvi[d] - t.band_base;
+        t.oklo[d] = hi >= lo ? (unsigned)lo : 0x80000000u;
+        t.okspan[d] = hi >= lo ? (unsigned)(hi - lo) : 0u;
     }
+    t.keep = pic.cvals != nullptr;
     t.plane_rs = buffer_over(pic.cvals ? pic.cvals + pic.off[s] : nullptr, pic.cvals ? (unsigned long long)t.w * t.h * 4 : 0);
     t.seg_rs = buffer_over(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h, (unsigned long long)kCvTileW * tile_h * 4);
     unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
