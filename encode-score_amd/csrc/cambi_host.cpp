@@ -173,6 +173,9 @@ void CambiUnit::plane(int s, uint16_t *host) const
     require(s >= 0 && s < kScales && host, "bad scale or output");
     hip_check(hipStreamSynchronize(own_), "cambi");
     hip_check(hipMemcpy(host, planes_.as<uint16_t>() + info_.off[s], (size_t)info_.n[s] * 2, hipMemcpyDeviceToHost), "reading a plane");
+    // the device holds codes; the caller gets value | mask << 15
+    const unsigned bias = plane_bias(info_.band_base);
+    for (long long i = 0; i < info_.n[s]; i++) host[i] = (uint16_t)(host[i] + bias);
 }
 
 void CambiUnit::cvalues(int s, float *host) const

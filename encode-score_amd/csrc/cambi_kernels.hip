@@ -2,7 +2,8 @@
 //
 // The per-picture pipeline of Cambi::launch / Cambi::finish (src/cambi.cpp:88-184) as four kernel kinds:
 //   cambi_base     scale 0: the spatial mask of the unfiltered picture (src/kernels.cu:9-40) and the 3x3 mode filter
-//                  (src/kernels.cu:486-500) by wave strips, packed as value | mask << 15;
+//                  (src/kernels.cu:486-500) by wave strips, each pixel stored as its code (value | mask << 15 less
+//                  a per-picture bias, so that a c-value tile indexes its tables with the code itself);
 //   cambi_scale    scales 1-4: decimation of the previous packed scale (src/kernels.cu:42-53) and the mode filter;
 //   cambi_cvalues  c-values of every scale (src/kernels.cu:129-278): per 64-column tile one wave keeps the window's
 //                  column counts of the tile's relevant values in LDS and turns them into window counts with a
@@ -116,6 +117,7 @@ __device__ __forceinline__ void base_strip(const CambiPic &pic, const Rd &rd, in
     // the packed plane as a buffer; a lane right of the picture stores past its end, where the store is dropped
     const __amdgpu_buffer_rsrc_t out = buffer_over(pic.planes, (unsigned long long)w * h * 2);
     const unsigned vo = x < w ? 2u * (unsigned)x : 0x80000000u;
+    const unsigned bias = plane_bias(pic.consts->band_base);
     // samples of rows r, r + 1 and r + 2 (loading) in slots (r - y0 + 3) mod 3 of A (columns a), B (columns b) and L
     // (own column); the row loop is unrolled 3 times, so the slots rotate by renaming
     int A[3], B[3], L[3] = {0, 0, 0};
@@ -189,7 +191,7 @@ __device__ __forceinline__ void base_strip(const CambiPic &pic, const Rd &rd, in
         prev = cc;
         if (store) {  // the row whose 7x7 count is complete, r - 3; its mode is the low half of md
             const unsigned y = (unsigned)(r - 3);
-            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(md | ((sum > index ? 1u : 0u) << 15)), out, vo,
+            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)((md | (sum > index ? 1u : 0u) << 15) - bias), out, vo,
                                                   __builtin_amdgcn_readfirstlane(2u * y * (unsigned)w), 0);
         }
         md = __builtin_amdgcn_alignbit((unsigned)m, md, 16);  // (m : md) >> 16: row r - 2's mode low, row r - 1's high
@@ -281,8 +283,10 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
     // byte offsets of the decimated samples (2x, 2y) of the previous scale
     const unsigned vc = 4u * (unsigned)min(x, w - 1), vh = 4u * (unsigned)(lane == 0 ? max(x0 - 1, 0) : min(x0 + 64, w - 1));
     const unsigned vo = x < w ? 2u * (unsigned)x : 0x80000000u;
+    // samples as value | mask << 15 (the codes with the bias added back)
+    const unsigned bias = plane_bias(pic.consts->band_base);
     auto ld = [&](unsigned v, int r) {
-        return (unsigned)__builtin_amdgcn_raw_buffer_load_b16(in, v, (unsigned)(2 * min(max(r, 0), h - 1)) * (unsigned)pw * 2u, 0);
+        return ((unsigned)__builtin_amdgcn_raw_buffer_load_b16(in, v, (unsigned)(2 * min(max(r, 0), h - 1)) * (unsigned)pw * 2u, 0) + bias) & 0xFFFFu;
     };
     // the horizontal 3-mode of a row from its own and halo samples; the picture's first and last columns keep theirs
     auto hmode = [&](unsigned c, unsigned hv) -> unsigned {
@@ -307,7 +311,7 @@ __global__ __launch_bounds__(256) void cambi_scale_kernel(const CambiPic *pics, 
         E[i] = ld(vh, y + 4);
         const unsigned t2 = hmode(c2, e2);
         const unsigned m = (y == 0 || y == h - 1) ? (c1 & 0x7FFFu) : mode3(t0, t1, t2);
-        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(m | (c1 & 0x8000u)), out, vo,
+        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)((m | (c1 & 0x8000u)) - bias), out, vo,
                                               __builtin_amdgcn_readfirstlane(2u * (unsigned)y * (unsigned)w), 0);
         t0 = t1;
         t1 = t2;
@@ -371,8 +375,9 @@ struct CvTile {
     int w, h, x0, y0, rows;
     int pad, band_base, band_size, vlt;
     int tvi[kDiffs], weight[kDiffs];
-    // the thresholds of difference d as one unsigned range test on a sample's index i (value less band_base plus 4):
-    // value + 4 <= tvi[d] and value + 4 + d + 1 > vlt exactly when i - oklo[d] <= okspan[d]; an empty range has
+    // the thresholds of difference d as one unsigned range test on a served sample's index i (its code, value less
+    // band_base plus 5): value + 4 <= tvi[d] and value + 4 + d + 1 > vlt exactly when i - oklo[d] <= okspan[d]; an
+    // empty range has
     // oklo = 2^31 and okspan = 0, which no index below 2^31 passes
     unsigned oklo[kDiffs], okspan[kDiffs];
     int keep;  // the full c-value plane is kept
@@ -391,10 +396,10 @@ __device__ __forceinline__ unsigned cv_at(const CvTile &t, unsigned xo, int y)
     return __builtin_amdgcn_raw_buffer_load_b16(t.rs, xo + (unsigned)(y * t.w * 2), 0, 0);
 }
 
-// A packed sample less (mask bit | band_base): the value less band_base when the sample is masked and not below the
-// band; an unmasked sample (or the 0 read outside the picture) wraps far above any band size.  So the sample is
-// active exactly when the result is below band_size.
-__device__ __forceinline__ unsigned cv_off(const CvTile &t, unsigned v) { return v - (0x8000u + (unsigned)t.band_base); }
+// A sample's code less kDiffs + 1: the value less band_base when the sample is masked and not below the band; an
+// unmasked sample, a masked one below the band or the 0 read outside the picture wraps far above any band size.  So the
+// sample is active exactly when the result is below band_size.
+__device__ __forceinline__ unsigned cv_off(const CvTile &, unsigned v) { return v - (unsigned)(kDiffs + 1); }
 
 // Index into the rank table of a packed sample: value - band_base + kDiffs when it is active, else 0 (whose entry is
 // kNone: no value below the band is relevant).
@@ -620,8 +625,8 @@ struct CvSharedM {
     alignas(16) uint8_t cnt[32 * NB][kMCnt];  // column counts of number n at margin column c
     uint16_t win[32 * NB + 1][kMWin];  // window counts of number n at output column x; row B: zeros
     // the pass's row of each value (its number less b0, or B outside the pass) at row[4 + i], i the value's index
-    // (value less band_base plus 4), with B at 0 .. 3 and from 4 + nv on, so that every index from 0 to nv and the 4
-    // values either side of it are entries
+    // (its code clamped to nv + 1: value less band_base plus 5 for a masked in-band value), with B at 0 .. 4 and from
+    // 5 + nv on, so that every index from 0 to nv + 1 and the 4 either side of it are entries
     uint8_t row[kMaxBand + 6 * kDiffs + 4];
     unsigned long long relevant[kMChunks];  // bit i of chunk j: value 64 j + i is relevant (numbered in value order)
     __device__ uint8_t *flags() { return (uint8_t *)&win[0][0]; }
@@ -643,9 +648,9 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 // touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[]), its number less b0, or B
 // when it is outside the pass or not relevant: window counts of such values read the zero row B of win, and their
 // count updates go to row B of cnt, which is unused or (past the count table) the first 128 bytes of win's row 0,
-// which every served row rewrites in full before it reads any of it.  A packed sample's index is
-// min(sample - (0x8000 + band_base - 4), nv): value - band_base + 4 for a masked sample at or above band_base - 4 and
-// below the band's end, nv (or more, then clamped) otherwise; every index outside 4 .. nv - 5 has row B.
+// which every served row rewrites in full before it reads any of it.  A sample's index is its code clamped to nv + 1
+// (one instruction): value - band_base + 5 for a masked sample in the band, and every index outside 5 .. nv - 4 has
+// row B (0 for a sample outside the picture).
 template <int NB, int NBU, int NJ>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c, unsigned &tcount)
 {
@@ -680,9 +685,9 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
     typedef __attribute__((address_space(3))) uint32_t lds_word;
     lds_word *wrow = (lds_word *)&sh.win[r32][4 * g];
     asm volatile("" : "+v"(wrow));
-    const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs), ibase = 0x8000u + (unsigned)t.band_base - kDiffs;
-    auto index = [&](unsigned v) { return min(v - ibase, nv); };
-    const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the row of index i (i from -4 to nv + 4)
+    const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs);
+    auto index = [&](unsigned v) { return min(v, nv + 1); };
+    const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the row of index i (i from -4 to nv + 5)
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
         for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
@@ -692,11 +697,11 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             const unsigned long long m = sh.relevant[j];
             const unsigned n = (unsigned)below + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
             const int i = j * 64 + lane;
-            if (i < (int)nv) sh.row[kDiffs + i] = (uint8_t)((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B);
+            if (i < (int)nv) sh.row[kDiffs + 1 + i] = (uint8_t)((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B);
             below += __popcll(m);
         }
-        if (lane < kDiffs) sh.row[lane] = (uint8_t)B;
-        if (lane < 2 * kDiffs + 1) sh.row[kDiffs + nv + lane] = (uint8_t)B;
+        if (lane <= kDiffs) sh.row[lane] = (uint8_t)B;
+        if (lane < 2 * kDiffs + 1) sh.row[kDiffs + 1 + nv + lane] = (uint8_t)B;
         __syncthreads();  // one wave: orders the table writes before the reads below
         const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
         for (int y = ya; y < yb; y += 8) {
@@ -817,7 +822,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
         };
         unsigned ca[5], cb[5] = {0, 0, 0, 0, 0};
         load(t.y0, ca);
-        // the first row removes nothing: index(0) is nv, whose row is B
+        // the first row removes nothing: index(0) is 0, whose row is B
         step(0, ca, cb, 0u, 0u);
         for (int r = 1; r < t.rows; r += 2) {
             step(r, cb, ca, ca[3], ca[4]);
@@ -915,8 +920,8 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     for (int d = 0; d < kDiffs; d++) {
         t.tvi[d] = __builtin_amdgcn_readfirstlane(K.tvi[d]);
         t.weight[d] = __builtin_amdgcn_readfirstlane(K.weight[d]) & 0xFF;  // 1 .. 4: 24-bit products stay exact
-        // i + band_base <= tvi[d] and i + band_base + d + 1 > vlt, with i >= 0
-        const int lo = max(t.vlt - t.band_base - d, 0), hi = t.tvi[d] - t.band_base;
+        // i - 1 + band_base <= tvi[d] and i - 1 + band_base + d + 1 > vlt, with i >= 1
+        const int lo = max(t.vlt - t.band_base - d, 0) + 1, hi = t.tvi[d] - t.band_base + 1;
         t.oklo[d] = hi >= lo ? (unsigned)lo : 0x80000000u;
         t.okspan[d] = hi >= lo ? (unsigned)(hi - lo) : 0u;
     }

@@ -47,7 +47,13 @@ struct PlaneIn {
     int rshift;
 };
 
-// One picture for the CAMBI kernels.  The packed scale planes hold value | mask << 15 per pixel, scale s at
+// The code of a packed CAMBI sample: (value | mask << 15) - plane_bias(band_base) modulo 2^16.  It is 1 + (value -
+// band_base + 4) for a masked value from band_base - 4 on, 0 for band_base - 5 and for what a buffer load past a
+// plane returns, and far above the band for an unmasked value or a masked one below band_base - 5; adding the bias
+// back gives value | mask << 15.
+__host__ __device__ constexpr unsigned plane_bias(int band_base) { return 0x8000u + (unsigned)band_base - (kDiffs + 1); }
+
+// One picture for the CAMBI kernels.  The packed scale planes hold each pixel's code (above), scale s at
 // planes + off[s] (and the optional full c-value planes at cvals + off[s]).  The positive c-values of a scale go to
 // per-tile segments: tile j of scale s (64 columns by the launch's tile height) writes its values from
 // list + loff[s] + j * 64 * tile_h and their number to seg_count[toff[s] + j], so no tile waits on another.
