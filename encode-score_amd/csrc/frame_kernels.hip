@@ -20,23 +20,22 @@
 namespace es {
 
 // 2^23 sqrt(s) rounded to single precision (round to nearest), as an exact integer, for 0 <= s < 2^24.  The hardware
-// square root may be one unit in the last place off; the candidate is corrected by comparing s with the squares of
-// the midpoints to its neighbours in exact integer arithmetic.  With f = F 2^(e - 150) (F the 24-bit significand, e
-// the biased exponent) the midpoints are (4F +- 2) 2^(e - 152), or (4F - 1) 2^(e - 152) below a power of two, so
-// s lies above the upper one when s 2^(304 - 2e) > (4F + 2)^2.  No square root of an integer lies on a midpoint.
+// square root r is within one unit in the last place; it is corrected by comparing s with the squares of the midpoints
+// to its neighbours.  A midpoint has a 25-bit significand, so it and its square (50 bits) are exact in double, as is
+// s.  In r's double the 29 significand bits below the float's 23 are zero: r + ulp / 2 is that pattern with bit 28
+// set, and the pattern less 2^28 is the midpoint below, r - ulp / 2 (or, when r is a power of two, r - ulp / 4: the
+// borrow crosses into the binade below, whose units are half as large).  No square root of an integer lies on a
+// midpoint.  Branch-free: s = 0 gives 0 (the lower pattern is then a NaN, which compares false).
 __device__ __forceinline__ unsigned long long strength_fixed(unsigned s)
 {
-    if (s == 0) return 0;
-    unsigned b = __float_as_uint(__builtin_sqrtf((float)s));
-    {
-        const unsigned e = b >> 23, F = (b & 0x7FFFFFu) | 0x800000u;
-        const unsigned long long t = (unsigned long long)s << (304 - 2 * e);
-        const unsigned long long up = 4ull * F + 2, dn = F == 0x800000u ? 4ull * F - 1 : 4ull * F - 2;
-        if (t > up * up) b++;
-        else if (t < dn * dn) b--;
-    }
+    unsigned b = __float_as_uint(__builtin_amdgcn_sqrtf((float)s));
+    const unsigned long long rb = (unsigned long long)__double_as_longlong((double)__uint_as_float(b));
+    const double hi = __longlong_as_double((long long)(rb | (1ull << 28)));
+    const double lo = __longlong_as_double((long long)(rb - (1ull << 28)));
+    const double sd = (double)s;
+    b = b + (sd > __dmul_rn(hi, hi) ? 1u : 0u) - (sd < __dmul_rn(lo, lo) ? 1u : 0u);
     const unsigned e = b >> 23, F = (b & 0x7FFFFFu) | 0x800000u;
-    return (unsigned long long)F << (e - 127);
+    return s ? (unsigned long long)F << (e - 127) : 0ull;
 }
 
 // The colour figures of one chroma sample pair at the 10-bit scale.
@@ -46,7 +45,7 @@ __device__ __forceinline__ void color_add(unsigned u, unsigned v, unsigned long 
     const int du = (int)u - 512, dv = (int)v - 512;
     su += u;
     sv += v;
-    ss += strength_fixed((unsigned)(du * du + dv * dv));
+    ss += strength_fixed((unsigned)(__mul24(du, du) + __mul24(dv, dv)));  // |du|, |dv| <= 512 at the 10-bit scale
 }
 
 // Adds N per-thread sums of a 256-thread block to out[0 .. N) with one atomic per nonzero block total.
@@ -129,8 +128,8 @@ static int sum_blocks(size_t n)
 
 __device__ __forceinline__ unsigned sq_diff(unsigned a, unsigned b)
 {
-    const int d = (int)a - (int)b;
-    return (unsigned)d * (unsigned)d;  // below 2^32 for 16-bit samples
+    const unsigned d = __usad(a, b, 0u);  // |a - b| < 2^16: an unsigned 24-bit product, below 2^32
+    return __umul24(d, d);
 }
 
 // vec: a and b (or u and v) are 16-byte aligned; then eight samples per load, the remainder one at a time.
