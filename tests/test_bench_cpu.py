@@ -45,7 +45,8 @@ def test_config3_and_5_keep_the_reference_proportions(es):
 
 
 @pytest.mark.parametrize("cfg,w,h,stage", [(1, 1920, 1080, "cambi_cvalues"), (2, 3840, 2160, "cambi_cvalues"),
-                                          (2, 3840, 2160, "window_stats"), (2, 3840, 2160, "persist")])
+                                          (2, 3840, 2160, "window_stats"), (2, 3840, 2160, "persist"),
+                                          (3, 1920, 1080, "cambi_cvalues"), (4, 1920, 1080, "cambi_cvalues")])
 def test_pmc_summaries_hold_the_profiled_kernels(cfg, w, h, stage):
     """The committed rocprofv3 summaries the bench's roofline reads: HBM bytes, counters and the profiled run's input
     bytes per launch for each kernel it may name."""
