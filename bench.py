@@ -312,7 +312,7 @@ def main():
           f"{batch})", file=sys.stderr, flush=True)
 
     # The input bytes of the timed frame steps (the algorithmic bytes of the path): each luma plane read once, the
-    # source once per step and every active encode's plane (SURVEY.md's B_in); with chroma, 1.5 times that.
+    # source once per step and every active encode's plane (B_in); with chroma, 1.5 times that.
     def active(f):
         return [d for d, length in encs if f < length]
 
