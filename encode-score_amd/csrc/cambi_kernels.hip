@@ -349,8 +349,9 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // runs one prefix sum per pair of numbers (16 bits each) over the column pairs, the scans of all pairs interleaved
 // step by step; the prefixes P(c) of every column go to LDS and a served pixel at output column x reads the window
 // count of each number it needs as P(x + span) - P(x).  The codes of the next row are loaded while the current row is
-// processed, the setup's loads go out eight rows at a time, and the c-value arithmetic is predicated rather than
-// branched.  Positive c-values go to the tile's own segment of the list at a wave-uniform offset.
+// processed, the setup's loads go out 32 at a time (a lane's samples of 32 rows, or of 16 rows of both its columns),
+// and the c-value arithmetic is predicated rather than branched.  Positive c-values go to the tile's own segment of
+// the list at a wave-uniform offset.
 // ---------------------------------------------------------------------------------------------------------------
 constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
 constexpr unsigned kCvDummy = kCvFlags - 1;  // the flag an inactive sample sets: past every index a tile reads
@@ -944,27 +945,28 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         return active;
     };
     bool any = false;
-    for (int r = 0; r < t.rows; r += 8) {
-        unsigned v[8];
+    // the setup waits on memory once per batch: 32 loads per lane in flight (the row loops below keep fewer)
+    for (int r = 0; r < t.rows; r += 32) {
+        unsigned v[32];
 #pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = cv_at(t, cv_col(t, t.x0 + lane), r + j < t.rows ? t.y0 + r + j : -1);
+        for (int j = 0; j < 32; j++) v[j] = cv_at(t, cv_col(t, t.x0 + lane), r + j < t.rows ? t.y0 + r + j : -1);
 #pragma unroll
-        for (int j = 0; j < 8; j++) any |= flag(served, v[j]);
+        for (int j = 0; j < 32; j++) any |= flag(served, v[j]);
     }
     if (!__any(any)) {  // most tiles of most pictures: nothing masked inside the band
         if (lane == 0) *seg_count = 0;
         return;
     }
     const int ya = max(t.y0 - t.pad, 0), yb = min(t.y0 + t.rows + t.pad, t.h);
-    for (int y = ya; y < yb; y += 8) {
-        unsigned v[16];
+    for (int y = ya; y < yb; y += 16) {
+        unsigned v[32];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
+        for (int j = 0; j < 16; j++) {
             v[2 * j] = cv_at(t, c0 < cw ? cv_col(t, xl) : kCvOut, y + j < yb ? y + j : -1);
             v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
         }
 #pragma unroll
-        for (int j = 0; j < 16; j++) flag(occupied, v[j]);
+        for (int j = 0; j < 32; j++) flag(occupied, v[j]);
     }
     __syncthreads();
     // the relevant values, 64 per chunk, lane c holding chunk c: occupied and within 4 of a served value (the served
