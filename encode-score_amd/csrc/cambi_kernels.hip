@@ -958,15 +958,38 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         return;
     }
     const int ya = max(t.y0 - t.pad, 0), yb = min(t.y0 + t.rows + t.pad, t.h);
-    for (int y = ya; y < yb; y += 16) {
-        unsigned v[32];
+    if ((t.w & 1) == 0 && ((uintptr_t)(pic.planes + pic.off[s]) & 3u) == 0) {
+        // An even width on a 4-byte aligned plane keeps every row's samples at even picture columns 4-byte aligned: a
+        // lane reads the two codes of one aligned pair in one load, window columns 2 lane - sh and 2 lane + 1 - sh
+        // (sh = 1 when the window starts at an odd column, and its column -1 is then left out).  A pair is inside or
+        // outside the picture as a whole (picture columns 2k, 2k + 1).
+        const int sh_ = (t.x0 - t.pad) & 1, cl = 2 * lane - sh_, xlo = t.x0 - t.pad + cl;
+        const unsigned xo = xlo >= 0 && xlo < t.w ? 2u * (unsigned)xlo : kCvOut;
+        const bool lo_ok = cl >= 0 && cl < cw, hi_ok = cl + 1 < cw;
+        for (int y = ya; y < yb; y += 32) {
+            unsigned v[32];
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-            v[2 * j] = cv_at(t, c0 < cw ? cv_col(t, xl) : kCvOut, y + j < yb ? y + j : -1);
-            v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
+            for (int j = 0; j < 32; j++) {
+                const int yy = y + j < yb ? y + j : -1;
+                v[j] = __builtin_amdgcn_raw_buffer_load_b32(t.rs, xo + (unsigned)(yy * t.w * 2), 0, 0);
+            }
+#pragma unroll
+            for (int j = 0; j < 32; j++) {
+                flag(occupied, lo_ok ? v[j] & 0xFFFFu : 0u);
+                flag(occupied, hi_ok ? v[j] >> 16 : 0u);
+            }
         }
+    } else {
+        for (int y = ya; y < yb; y += 16) {
+            unsigned v[32];
 #pragma unroll
-        for (int j = 0; j < 32; j++) flag(occupied, v[j]);
+            for (int j = 0; j < 16; j++) {
+                v[2 * j] = cv_at(t, c0 < cw ? cv_col(t, xl) : kCvOut, y + j < yb ? y + j : -1);
+                v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
+            }
+#pragma unroll
+            for (int j = 0; j < 32; j++) flag(occupied, v[j]);
+        }
     }
     __syncthreads();
     // the relevant values, 64 per chunk, lane c holding chunk c: occupied and within 4 of a served value (the served
