@@ -381,7 +381,8 @@ struct CvTile {
     // empty range has
     // oklo = 2^31 and okspan = 0, which no index below 2^31 passes
     unsigned oklo[kDiffs], okspan[kDiffs];
-    int keep;  // the full c-value plane is kept
+    int keep;   // the full c-value plane is kept
+    int pairs;  // an even width on a 4-byte aligned plane: every row's codes at even columns are 4-byte aligned
     // the full c-value plane (when kept; else an empty buffer) and this tile's segment of the list, as buffers
     __amdgpu_buffer_rsrc_t plane_rs, seg_rs;
 };
@@ -652,16 +653,22 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 // which every served row rewrites in full before it reads any of it.  A sample's index is its code clamped to nv + 1
 // (one instruction): value - band_base + 5 for a masked sample in the band, and every index outside 5 .. nv - 4 has
 // row B (0 for a sample outside the picture).
-template <int NB, int NBU, int NJ>
+template <int NB, int NBU, int NJ, bool kPairs>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c, unsigned &tcount)
 {
     static_assert(NBU >= 1 && NBU <= NB && NJ >= 1 && NJ <= 3, "blocks of numbers, band blocks");
     constexpr int B = 32 * NBU;
-    const int lane = threadIdx.x, pad = t.pad, span = 2 * pad;
-    const int c0 = 2 * lane, xl = t.x0 - pad + c0;
-    const bool col0 = c0 < kCvTileW + 2 * pad, col1 = c0 + 1 < kCvTileW + 2 * pad;
+    const int lane = threadIdx.x, pad = t.pad, span = 2 * pad, cwid = kCvTileW + 2 * pad;
+    // this lane's two margin columns c0 and c0 + 1: 2 lane and 2 lane + 1, or with kPairs the 4-byte aligned pair of
+    // picture columns, 2 lane - sh and 2 lane + 1 - sh (sh = 1 when the window starts at an odd column), read as one
+    // load.  A column outside the window (-1, or cwid and up) is read but never counted: its increment is 0.
+    const int sh_ = kPairs ? ((t.x0 - pad) & 1) : 0;
+    const int c0 = 2 * lane - sh_, xl = t.x0 - pad + c0;
+    const bool col0 = c0 >= 0 && c0 < cwid, col1 = c0 + 1 < cwid;
     const int x = t.x0 + lane;
     const unsigned xo0 = col0 ? cv_col(t, xl) : kCvOut, xo1 = col1 ? cv_col(t, xl + 1) : kCvOut, xoo = cv_col(t, x);
+    // kPairs: the pair is inside or outside the picture as a whole (picture columns 2k, 2k + 1)
+    const unsigned xop = xl >= 0 && xl < t.w ? 2u * (unsigned)xl : kCvOut;
     // band operands of output block mb and margin block mb + j: this lane's row 32 mb + lane % 32 and its half of
     // the block's columns
     const int g = lane >> 5, r32 = lane & 31;
@@ -670,14 +677,27 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
     for (int mb = 0; mb < 2; mb++)
 #pragma unroll
         for (int j = 0; j < NJ; j++) band[mb][j] = band_bytes(32 * mb + r32, 32 * (mb + j) + 16 * g, span);
-    // a lane's two margin columns share a count word of every number's row: its byte offset in the row, and the
-    // increments of its columns' bytes (a decrement is the negated increment)
-    const unsigned cword = (unsigned)(c0 >> 2) * 4u;
-    const unsigned inc0 = 1u << ((c0 & 3) * 8), inc1 = 1u << (((c0 + 1) & 3) * 8);
+    // the count words of the lane's two margin columns in every number's row (the same word unless sh = 1): their
+    // byte offsets in the row, and the increments of the columns' bytes (a decrement is the negated increment; 0 for
+    // a column outside the window)
+    const unsigned cword0 = (unsigned)(max(c0, 0) >> 2) * 4u, cword1 = (unsigned)((c0 + 1) >> 2) * 4u;
+    const unsigned inc0 = col0 ? 1u << ((c0 & 3) * 8) : 0u, inc1 = col1 ? 1u << (((c0 + 1) & 3) * 8) : 0u;
     uint8_t *const cnt = &sh.cnt[0][0];
     static_assert(offsetof(CvSharedM<NB>, cnt) + 32 * NB * kMCnt == offsetof(CvSharedM<NB>, win) && kMWin * 2 >= 128,
                   "updates at row B past the count table land in win's row 0");
-    auto bump = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword), inc); };
+    auto bump0 = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword0), inc); };
+    auto bump1 = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword1), inc); };
+    // the codes of the lane's two margin columns in row y
+    auto ld2 = [&](int y, unsigned &a, unsigned &b) __attribute__((always_inline)) {
+        if (kPairs) {
+            const unsigned p = __builtin_amdgcn_raw_buffer_load_b32(t.rs, xop + (unsigned)(y * t.w * 2), 0, 0);
+            a = p & 0xFFFFu;
+            b = p >> 16;
+        } else {
+            a = cv_at(t, xo0, y);
+            b = cv_at(t, xo1, y);
+        }
+    };
     // this lane's start in the count table (B operand: number r32, its 16 margin columns of a block) and in the
     // window table (the products it writes: number r32, output columns 4 g ..)
     const uint8_t *const crow = &sh.cnt[r32][16 * g];
@@ -708,29 +728,23 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
         for (int y = ya; y < yb; y += 8) {
             unsigned v[16];
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int yy = y + j < yb ? y + j : -1;
-                v[2 * j] = cv_at(t, xo0, yy);
-                v[2 * j + 1] = cv_at(t, xo1, yy);
-            }
+            for (int j = 0; j < 8; j++) ld2(y + j < yb ? y + j : -1, v[2 * j], v[2 * j + 1]);
             unsigned rw[16];
 #pragma unroll
             for (int j = 0; j < 16; j++) rw[j] = rows[index(v[j])];
 #pragma unroll
             for (int j = 0; j < 8; j++) {
-                bump(rw[2 * j], inc0);
-                bump(rw[2 * j + 1], inc1);
+                bump0(rw[2 * j], inc0);
+                bump1(rw[2 * j + 1], inc1);
             }
         }
         // codes of a row: entering (y + pad), own (y), leaving (y - pad); loaded a row ahead into the other of two
         // arrays (the row loop is unrolled twice, so they swap roles by renaming).  The leaving samples of row y are
         // removed at the start of row y + 1, with that row's entering ones.
         auto load = [&](int y, unsigned *v) {
-            v[0] = cv_at(t, xo0, y + pad);
-            v[1] = cv_at(t, xo1, y + pad);
+            ld2(y + pad, v[0], v[1]);
             v[2] = cv_at(t, xoo, y);
-            v[3] = cv_at(t, xo0, y - pad);
-            v[4] = cv_at(t, xo1, y - pad);
+            ld2(y - pad, v[3], v[4]);
         };
         const unsigned lo_rel = (unsigned)(lo - b0), hi_rel = (unsigned)(hi - b0);
         auto step = [&](const int r, const unsigned *cur, unsigned *nxt, unsigned gone0, unsigned gone1) __attribute__((always_inline)) {
@@ -740,10 +754,10 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             const unsigned rl0 = rows[index(gone0)], rl1 = rows[index(gone1)];
             const unsigned own = rows[io];
             load(y + 1, nxt);  // past the tile's last row it reads unused samples (past the picture, zeros)
-            bump(rl0, 0u - inc0);
-            bump(rl1, 0u - inc1);
-            bump(re0, inc0);
-            bump(re1, inc1);
+            bump0(rl0, 0u - inc0);
+            bump1(rl1, 0u - inc1);
+            bump0(re0, inc0);
+            bump1(re1, inc1);
             const bool serve = own >= lo_rel && own < hi_rel;
             if (__builtin_amdgcn_ballot_w64(serve)) {
                 // rows of the own value and the 4 values either side, read only for a row the pass serves (they are
@@ -854,8 +868,13 @@ template <int NB, int NJ>
 __device__ __forceinline__ void cv_run_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c,
                                          unsigned &tcount)
 {
-    if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ>(sh, t, relevant, sum_c, tcount);
-    else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ>(sh, t, relevant, sum_c, tcount);
+    if (t.pairs) {
+        if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ, true>(sh, t, relevant, sum_c, tcount);
+        else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ, true>(sh, t, relevant, sum_c, tcount);
+    } else {
+        if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ, false>(sh, t, relevant, sum_c, tcount);
+        else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ, false>(sh, t, relevant, sum_c, tcount);
+    }
 }
 template <int NB>
 __device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c,
@@ -927,6 +946,7 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         t.okspan[d] = hi >= lo ? (unsigned)(hi - lo) : 0u;
     }
     t.keep = pic.cvals != nullptr;
+    t.pairs = (t.w & 1) == 0 && ((uintptr_t)(pic.planes + pic.off[s]) & 3u) == 0;
     t.plane_rs = buffer_over(pic.cvals ? pic.cvals + pic.off[s] : nullptr, pic.cvals ? (unsigned long long)t.w * t.h * 4 : 0);
     t.seg_rs = buffer_over(pic.list + pic.loff[s] + (long long)local * kCvTileW * tile_h, (unsigned long long)kCvTileW * tile_h * 4);
     unsigned *seg_count = pic.seg_count + pic.toff[s] + local;
@@ -958,7 +978,7 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         return;
     }
     const int ya = max(t.y0 - t.pad, 0), yb = min(t.y0 + t.rows + t.pad, t.h);
-    if ((t.w & 1) == 0 && ((uintptr_t)(pic.planes + pic.off[s]) & 3u) == 0) {
+    if (t.pairs) {
         // An even width on a 4-byte aligned plane keeps every row's samples at even picture columns 4-byte aligned: a
         // lane reads the two codes of one aligned pair in one load, window columns 2 lane - sh and 2 lane + 1 - sh
         // (sh = 1 when the window starts at an odd column, and its column -1 is then left out).  A pair is inside or
