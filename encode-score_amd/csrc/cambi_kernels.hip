@@ -966,12 +966,32 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
     };
     bool any = false;
     // the setup waits on memory once per batch: 32 loads per lane in flight (the row loops below keep fewer)
-    for (int r = 0; r < t.rows; r += 32) {
-        unsigned v[32];
+    if (t.pairs) {
+        // aligned pairs: lane l reads picture columns x0 + 2 (l % 32) and the next of row r + l / 32, so that one load
+        // covers two rows of the tile
+        const int xp = t.x0 + 2 * (lane & 31), rl = lane >> 5;
+        const unsigned xo = xp < t.w ? 2u * (unsigned)xp : kCvOut;
+        for (int r = 0; r < t.rows; r += 64) {
+            unsigned v[32];
 #pragma unroll
-        for (int j = 0; j < 32; j++) v[j] = cv_at(t, cv_col(t, t.x0 + lane), r + j < t.rows ? t.y0 + r + j : -1);
+            for (int j = 0; j < 32; j++) {
+                const int rr = r + 2 * j + rl;
+                v[j] = __builtin_amdgcn_raw_buffer_load_b32(t.rs, xo + (unsigned)((rr < t.rows ? t.y0 + rr : -1) * t.w * 2), 0, 0);
+            }
 #pragma unroll
-        for (int j = 0; j < 32; j++) any |= flag(served, v[j]);
+            for (int j = 0; j < 32; j++) {
+                any |= flag(served, v[j] & 0xFFFFu);
+                any |= flag(served, v[j] >> 16);
+            }
+        }
+    } else {
+        for (int r = 0; r < t.rows; r += 32) {
+            unsigned v[32];
+#pragma unroll
+            for (int j = 0; j < 32; j++) v[j] = cv_at(t, cv_col(t, t.x0 + lane), r + j < t.rows ? t.y0 + r + j : -1);
+#pragma unroll
+            for (int j = 0; j < 32; j++) any |= flag(served, v[j]);
+        }
     }
     if (!__any(any)) {  // most tiles of most pictures: nothing masked inside the band
         if (lane == 0) *seg_count = 0;
