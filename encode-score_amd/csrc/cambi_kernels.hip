@@ -97,8 +97,9 @@ struct BaseReadFrom8 {  // cambi.c's 8-bit anti-dithering mean; offsets are the 
 // One strip.  Positions outside the picture enter only as wave masks: margin columns outside it and rows outside it
 // have no flat bits, and the mode filter reads no sample outside the picture for any pixel it writes.  An interior
 // strip (kInterior: every row it reads lies inside the picture, rows 0, 1, h - 2 and h - 1 excepted) makes no row tests
-// and reads its rows at a running offset.  With kLoadOwn the lane's own column is loaded as well (cheap readers);
-// otherwise it is taken from the margin columns' samples by a lane swap.
+// and reads its rows at a running offset.  With kLoadOwn the lane's own column is loaded as well (the shifting
+// readers); otherwise it is taken from the margin columns' samples by a lane swap, one load fewer per row (the plain
+// 10-bit reader and the anti-dithering one).
 template <bool kLoadOwn, bool kInterior, typename Rd>
 __device__ __forceinline__ void base_strip(const CambiPic &pic, const Rd &rd, int x0, int y0, int y1)
 {
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256) void cambi_base_kernel(const CambiPic *pics)
         base_strip_any<true>(pic, BaseRead8{buffer_over(in.ptr, (unsigned long long)in.pitch * h), w, h, (unsigned)in.pitch, in.shift}, b.x);
     } else {
         const BaseRead16 r16{buffer_over(in.ptr, (unsigned long long)in.pitch * h * 2), w, h, 2u * (unsigned)in.pitch};
-        if (in.rshift == 0 && in.shift == 0) base_strip_any<true>(pic, r16, b.x);  // 10-bit samples, the common case
+        if (in.rshift == 0 && in.shift == 0) base_strip_any<false>(pic, r16, b.x);  // 10-bit samples, the common case
         else base_strip_any<true>(pic, BaseRead16S{r16, in.rshift, in.shift}, b.x);
     }
 }
