@@ -354,7 +354,7 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // and the c-value arithmetic is predicated rather than branched.  Positive c-values go to the tile's own segment of
 // the list at a wave-uniform offset.
 // ---------------------------------------------------------------------------------------------------------------
-constexpr int kCvFlags = 1040;  // >= kMaxBand + 2 * kDiffs, multiple of 16
+constexpr int kCvFlags = 592;   // >= kMaxBand + 2 * kDiffs, multiple of 16
 constexpr unsigned kCvDummy = kCvFlags - 1;  // the flag an inactive sample sets: past every index a tile reads
 constexpr int kCvCols = 130;    // prefix entries per number: columns 0 .. 128, padded to an even count
 
@@ -607,34 +607,43 @@ __device__ void cv_passes(Sh &sh, const CvTile &t, int relevant, double &sum_c, 
 // ---------------------------------------------------------------------------------------------------------------
 // c-values on the matrix cores.  The same tiles, numbering and passes; per row the window count of every number of
 // the pass at every output column comes from one banded matrix product instead of prefix scans:
-//   W[x][n] = sum over margin columns c of A[x][c] C[c][n],   A[x][c] = [x <= c <= x + 2 pad],
+//   W[n][x] = sum over margin columns c of C[n][c] A[c][x],   A[c][x] = [x <= c <= x + 2 pad],
 // with C the column counts (u8, at most 2 pad + 1) and W at most (2 pad + 1)^2, exact in i32.  The 64 output columns
 // are two blocks of 32 and the margin columns (64 + 2 pad <= 128) four blocks of 32; a block product is one
 // v_mfma_i32_32x32x32_i8 for 32 numbers, and only the blocks the band touches are multiplied (at most 3 per output
-// block).  A lane holds the band bytes of its output row and its 16 margin columns of each block, built once per
-// tile; the B operand is read 16 bytes at a time from a [number][column] byte table of the counts.  The products go
-// to a [number][column] u16 table from which each served pixel reads the counts of its numbers.  The lane layout of
-// the K dimension is the same for both operands (lane / 32 picks the 16-column half of the block), so the sum over
-// K does not depend on the hardware's order of it.
+// block).  The counts are the A operand, read 16 bytes at a time from a [number][column] byte table; a lane holds the
+// band bytes of its output column over 16 margin columns of each block (the B operand), built once per tile.  The
+// lane layout of the K dimension is the same for both operands (lane / 32 picks the 16-column half of the block), so
+// the sum over K does not depend on the hardware's order of it.  The product leaves each lane the counts of 16
+// numbers at one output column; they go to a [column][number] u16 table, one row per output column, from which the
+// served pixel of that column reads the counts of its numbers.
 // ---------------------------------------------------------------------------------------------------------------
 typedef int cv_v4i __attribute__((ext_vector_type(4)));
 typedef int cv_v16i __attribute__((ext_vector_type(16)));
 constexpr int kMCnt = 144;  // bytes per number in the count table: 128 columns, rows 16-byte aligned on spread banks
-constexpr int kMWin = 66;   // u16 per number in the window table: 64 columns and 2 of spread
 constexpr int kMChunks = (kMaxBand + 2 * kDiffs + 63) / 64;  // 64-value chunks of the relevance masks
+
+// u16 entries per output column in the window table: the B numbers, a zero at B and one of spread, so that a row is
+// an odd number of 4-byte words (17 for B = 32): the 32 columns a half-wave writes or reads at one number fall in 32
+// distinct banks.
+template <int NB> constexpr int kMWinT = 32 * NB + 2;
 
 template <int NB>
 struct CvSharedM {
-    alignas(16) uint8_t cnt[32 * NB][kMCnt];  // column counts of number n at margin column c
-    uint16_t win[32 * NB + 1][kMWin];  // window counts of number n at output column x; row B: zeros
-    // the pass's row of each value (its number less b0, or B outside the pass) at row[4 + i], i the value's index
-    // (its code clamped to nv + 1: value less band_base plus 5 for a masked in-band value), with B at 0 .. 4 and from
-    // 5 + nv on, so that every index from 0 to nv + 1 and the 4 either side of it are entries
-    uint8_t row[kMaxBand + 6 * kDiffs + 4];
+    // column counts of number n at margin column c; row B takes the updates of samples outside the pass
+    alignas(16) uint8_t cnt[32 * NB + 1][kMCnt];
+    uint16_t win[kCvTileW][kMWinT<NB>];  // window counts at output column x of number n; entry B: zero
+    // twice the pass's row of each value (twice its number less b0, or 2 B outside the pass or not relevant) at
+    // row[4 + i], i the value's index (its code clamped to nv + 1: value less band_base plus 5 for a masked in-band
+    // value), with 2 B at 0 .. 4 and from 5 + nv on, so that every index from 0 to nv + 1 and the 4 either side of it
+    // are entries, and the three 4-byte words from row[i & ~3] hold row[i .. i + 8]
+    alignas(4) uint8_t row[kMaxBand + 6 * kDiffs + 4];
     unsigned long long relevant[kMChunks];  // bit i of chunk j: value 64 j + i is relevant (numbered in value order)
     __device__ uint8_t *flags() { return (uint8_t *)&win[0][0]; }
 };
-static_assert(sizeof(uint16_t) * 32 * kMWin >= 2 * kCvFlags, "flags live in the window table during the setup");
+static_assert(sizeof(uint16_t) * kCvTileW * kMWinT<1> >= 2 * kCvFlags, "flags live in the window table during the setup");
+static_assert(sizeof(CvSharedM<1>) <= 10240, "16 waves of one CU share its 160 KB of LDS");
+static_assert(2 * 32 * 2 < 256, "twice a row fits a byte");
 
 // The 16 band bytes [xr <= c <= xr + span] over margin columns c = cb .. cb + 15, one byte per column.
 __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
@@ -648,12 +657,11 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 }
 
 // Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds); the band of every output block
-// touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[]), its number less b0, or B
-// when it is outside the pass or not relevant: window counts of such values read the zero row B of win, and their
-// count updates go to row B of cnt, which is unused or (past the count table) the first 128 bytes of win's row 0,
-// which every served row rewrites in full before it reads any of it.  A sample's index is its code clamped to nv + 1
-// (one instruction): value - band_base + 5 for a masked sample in the band, and every index outside 5 .. nv - 4 has
-// row B (0 for a sample outside the picture).
+// touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[], doubled): its number less
+// b0, or B when it is outside the pass or not relevant.  Window counts of such values read the zero entry B of the
+// pixel's column, and their count updates go to row B of cnt, which nothing reads.  A sample's index is its code
+// clamped to nv + 1 (one instruction): value - band_base + 5 for a masked sample in the band, and every index outside
+// 5 .. nv - 4 has row B (0 for a sample outside the picture).
 template <int NB, int NBU, int NJ, bool kPairs>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c, unsigned &tcount)
 {
@@ -670,8 +678,8 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
     const unsigned xo0 = col0 ? cv_col(t, xl) : kCvOut, xo1 = col1 ? cv_col(t, xl + 1) : kCvOut, xoo = cv_col(t, x);
     // kPairs: the pair is inside or outside the picture as a whole (picture columns 2k, 2k + 1)
     const unsigned xop = xl >= 0 && xl < t.w ? 2u * (unsigned)xl : kCvOut;
-    // band operands of output block mb and margin block mb + j: this lane's row 32 mb + lane % 32 and its half of
-    // the block's columns
+    // band operands of output block mb and margin block mb + j: this lane's output column 32 mb + lane % 32 and its
+    // half of the block's margin columns
     const int g = lane >> 5, r32 = lane & 31;
     cv_v4i band[2][NJ];
 #pragma unroll
@@ -684,10 +692,9 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
     const unsigned cword0 = (unsigned)(max(c0, 0) >> 2) * 4u, cword1 = (unsigned)((c0 + 1) >> 2) * 4u;
     const unsigned inc0 = col0 ? 1u << ((c0 & 3) * 8) : 0u, inc1 = col1 ? 1u << (((c0 + 1) & 3) * 8) : 0u;
     uint8_t *const cnt = &sh.cnt[0][0];
-    static_assert(offsetof(CvSharedM<NB>, cnt) + 32 * NB * kMCnt == offsetof(CvSharedM<NB>, win) && kMWin * 2 >= 128,
-                  "updates at row B past the count table land in win's row 0");
-    auto bump0 = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword0), inc); };
-    auto bump1 = [&](unsigned row, unsigned inc) { atomicAdd((uint32_t *)(cnt + row * kMCnt + cword1), inc); };
+    // a doubled row r2 = 2 row: the row's bytes start at r2 * kMCnt / 2
+    auto bump0 = [&](unsigned r2, unsigned inc) { atomicAdd((uint32_t *)(cnt + r2 * (kMCnt / 2) + cword0), inc); };
+    auto bump1 = [&](unsigned r2, unsigned inc) { atomicAdd((uint32_t *)(cnt + r2 * (kMCnt / 2) + cword1), inc); };
     // the codes of the lane's two margin columns in row y
     auto ld2 = [&](int y, unsigned &a, unsigned &b) __attribute__((always_inline)) {
         if (kPairs) {
@@ -699,31 +706,33 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             b = cv_at(t, xo1, y);
         }
     };
-    // this lane's start in the count table (B operand: number r32, its 16 margin columns of a block) and in the
-    // window table (the products it writes: number r32, output columns 4 g ..)
+    // this lane's start in the count table (A operand: number r32, its 16 margin columns of a block)
     const uint8_t *const crow = &sh.cnt[r32][16 * g];
-    // the window table's words from this lane's first product on, as one LDS address held in a register, so that
-    // every store of the products is that address plus a small immediate offset
+    // the window table's words of the products this lane writes (output column 32 mb + r32, numbers 4 g ..), as one
+    // LDS address per output block held in a register, so that every store is that address plus an immediate offset
     typedef __attribute__((address_space(3))) uint32_t lds_word;
-    lds_word *wrow = (lds_word *)&sh.win[r32][4 * g];
-    asm volatile("" : "+v"(wrow));
+    lds_word *wcol[2] = {(lds_word *)&sh.win[r32][4 * g], (lds_word *)&sh.win[32 + r32][4 * g]};
+    asm volatile("" : "+v"(wcol[0]), "+v"(wcol[1]));
+    // the row of this lane's own output column, read by its served pixel at byte offset 2 n for number n
+    typedef __attribute__((address_space(3))) const uint8_t lds_byte;
+    lds_byte *const wmine = (lds_byte *)&sh.win[lane][0];
     const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs);
     auto index = [&](unsigned v) { return min(v, nv + 1); };
-    const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the row of index i (i from -4 to nv + 5)
+    const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the doubled row of index i (i from -4 to nv + 5)
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
         for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
-        if (lane < kMWin / 2) ((uint32_t *)&sh.win[B][0])[lane] = 0u;
+        ((uint32_t *)&sh.win[lane][B])[0] = 0u;  // the zero entry B of every column (the setup's flags lived here)
         // the rows of the pass: a relevant value's number is the count of relevant values below it
         for (int j = 0, below = 0; j * 64 < (int)nv; j++) {
             const unsigned long long m = sh.relevant[j];
             const unsigned n = (unsigned)below + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
             const int i = j * 64 + lane;
-            if (i < (int)nv) sh.row[kDiffs + 1 + i] = (uint8_t)((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B);
+            if (i < (int)nv) sh.row[kDiffs + 1 + i] = (uint8_t)(2u * ((m >> lane & 1ull) ? min(n - (unsigned)b0, (unsigned)B) : (unsigned)B));
             below += __popcll(m);
         }
-        if (lane <= kDiffs) sh.row[lane] = (uint8_t)B;
-        if (lane < 2 * kDiffs + 1) sh.row[kDiffs + 1 + nv + lane] = (uint8_t)B;
+        if (lane <= kDiffs) sh.row[lane] = (uint8_t)(2 * B);
+        if (lane < 2 * kDiffs + 1) sh.row[kDiffs + 1 + nv + lane] = (uint8_t)(2 * B);
         __syncthreads();  // one wave: orders the table writes before the reads below
         const int ya = max(t.y0 - pad, 0), yb = min(t.y0 + pad, t.h);
         for (int y = ya; y < yb; y += 8) {
@@ -747,7 +756,7 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             v[2] = cv_at(t, xoo, y);
             ld2(y - pad, v[3], v[4]);
         };
-        const unsigned lo_rel = (unsigned)(lo - b0), hi_rel = (unsigned)(hi - b0);
+        const unsigned lo2 = 2u * (unsigned)(lo - b0), hi2 = 2u * (unsigned)(hi - b0);
         auto step = [&](const int r, const unsigned *cur, unsigned *nxt, unsigned gone0, unsigned gone1) __attribute__((always_inline)) {
             const int y = t.y0 + r;
             const unsigned io = index(cur[2]);
@@ -759,46 +768,69 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             bump1(rl1, 0u - inc1);
             bump0(re0, inc0);
             bump1(re1, inc1);
-            const bool serve = own >= lo_rel && own < hi_rel;
-            if (__builtin_amdgcn_ballot_w64(serve)) {
-                // rows of the own value and the 4 values either side, read only for a row the pass serves (they are
-                // not needed until after the products below)
-                unsigned rk[2 * kDiffs + 1];
-#pragma unroll
-                for (int k = 0; k < 2 * kDiffs + 1; k++) rk[k] = k == kDiffs ? own : rows[io - kDiffs + k];
+            const bool serve = own >= lo2 && own < hi2;
+            const unsigned long long sm = __builtin_amdgcn_ballot_w64(serve);
+            if (sm) {
                 // the LDS operations of one wave complete in order: the row's updates are in before these reads
                 asm volatile("" ::: "memory");
-                cv_v4i bop[NBU][NJ + 1];
+                // the doubled rows of the own value and the 4 values either side, rows[io - 4 .. io + 4] =
+                // row[io .. io + 8], as three aligned words, read only for a row the pass serves, after the products
+                // are issued (they are not needed until after the products are stored)
+                unsigned d0, d1, d2;
+                // the products of the output blocks some pixel of the row is served in (U0: columns 0 .. 31, U1:
+                // 32 .. 63), the two blocks' chains interleaved so that neither waits on its own previous product
+                auto products = [&](auto U0, auto U1) __attribute__((always_inline)) {
+                    constexpr bool use[2] = {decltype(U0)::value, decltype(U1)::value};
+                    constexpr int kb0 = use[0] ? 0 : 1, kb1 = use[1] ? NJ : NJ - 1;
+                    cv_v4i aop[NBU][NJ + 1];
 #pragma unroll
-                for (int nb = 0; nb < NBU; nb++)
+                    for (int nb = 0; nb < NBU; nb++)
 #pragma unroll
-                    for (int kb = 0; kb <= NJ; kb++) bop[nb][kb] = *(const cv_v4i *)(crow + (32 * nb * kMCnt + 32 * kb));
+                        for (int kb = kb0; kb <= kb1; kb++) aop[nb][kb] = *(const cv_v4i *)(crow + (32 * nb * kMCnt + 32 * kb));
 #pragma unroll
-                for (int nb = 0; nb < NBU; nb++) {
-                    // the two output blocks' products interleaved, so that neither waits on its own previous one
-                    cv_v16i acc[2];
-#pragma unroll
-                    for (int mb = 0; mb < 2; mb++) acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][0], bop[nb][mb], cv_v16i{}, 0, 0, 0);
-#pragma unroll
-                    for (int j = 1; j < NJ; j++)
+                    for (int nb = 0; nb < NBU; nb++) {
+                        cv_v16i acc[2];
 #pragma unroll
                         for (int mb = 0; mb < 2; mb++)
-                            acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(band[mb][j], bop[nb][mb + j], acc[mb], 0, 0, 0);
-                    // D's register 4 q + i of this lane: output column 32 mb + 8 q + 4 (lane / 32) + i, number
-                    // 32 nb + lane % 32; two columns per 4-byte store
+                            if (use[mb]) acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[nb][mb], band[mb][0], cv_v16i{}, 0, 0, 0);
 #pragma unroll
-                    for (int mb = 0; mb < 2; mb++)
+                        for (int j = 1; j < NJ; j++)
 #pragma unroll
-                        for (int q = 0; q < 4; q++)
+                            for (int mb = 0; mb < 2; mb++)
+                                if (use[mb]) acc[mb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(aop[nb][mb + j], band[mb][j], acc[mb], 0, 0, 0);
+                        if (nb == NBU - 1) {
+                            const uint32_t *const rw = (const uint32_t *)&sh.row[io & ~3u];
+                            d0 = rw[0];
+                            d1 = rw[1];
+                            d2 = rw[2];
+                        }
+                        // D's register 4 q + i of this lane: number 32 nb + 8 q + 4 (lane / 32) + i, output column
+                        // 32 mb + lane % 32; two numbers per 4-byte store
 #pragma unroll
-                            for (int e = 0; e < 2; e++)
-                                wrow[(32 * nb * kMWin + 32 * mb + 8 * q + 2 * e) / 2] =
-                                    (unsigned)acc[mb][4 * q + 2 * e] | (unsigned)acc[mb][4 * q + 2 * e + 1] << 16;
-                }
+                        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+                            for (int q = 0; q < 4; q++)
+#pragma unroll
+                                for (int e = 0; e < 2; e++)
+                                    if (use[mb])
+                                        wcol[mb][16 * nb + 4 * q + e] = (unsigned)acc[mb][4 * q + 2 * e] | (unsigned)acc[mb][4 * q + 2 * e + 1] << 16;
+                    }
+                };
+                typedef std::integral_constant<bool, true> On;
+                typedef std::integral_constant<bool, false> Off;
+                if ((unsigned)sm == 0u) products(Off(), On());
+                else if ((sm >> 32) == 0ull) products(On(), Off());
+                else products(On(), On());
+                // byte k of the nine: the doubled row of value index io - 4 + k
+                const unsigned w0 = __builtin_amdgcn_alignbyte(d1, d0, io), w1 = __builtin_amdgcn_alignbyte(d2, d1, io),
+                               w2 = __builtin_amdgcn_alignbyte(d2, d2, io);
                 asm volatile("" ::: "memory");
                 unsigned cnt9[2 * kDiffs + 1];
 #pragma unroll
-                for (int k = 0; k < 2 * kDiffs + 1; k++) cnt9[k] = sh.win[rk[k]][lane];
+                for (int k = 0; k < 2 * kDiffs + 1; k++) {
+                    const unsigned r2 = ((k < 4 ? w0 : k < 8 ? w1 : w2) >> (8 * (k & 3))) & 0xFFu;
+                    cnt9[k] = *(const __attribute__((address_space(3))) uint16_t *)(wmine + r2);
+                }
                 // the index of a served pixel, and past every threshold range on the other lanes, so that one range
                 // test per difference takes both the thresholds and the serving
                 const unsigned ios = serve ? io : 0x40000000u;

@@ -15,7 +15,9 @@ constexpr int kDiffs = 4;
 constexpr uint16_t kNone = 0xFFFF;
 constexpr int kCvTileW = 64;     // c-value tiles are one wave wide
 constexpr int kMaxPad = 32;      // one wave covers the tile and both margins as 2 columns per lane: window <= 65
-constexpr int kMaxBand = 1024;   // band values, 10-bit codes
+// band values: the display model fixes the band (band_base 0 to the threshold of difference 4 plus 5: 568 values of
+// 10-bit codes); the parameters' set-up checks it, and the c-value kernel's LDS tables are sized by it
+constexpr int kMaxBand = 576;
 constexpr int kXcds = 8;          // XCDs of an MI355X: a launch's workgroups go to them in turn
 constexpr int kHistBins = 20001; // per-window histograms: 0.05-point bins over [-500, 500]
 constexpr int kRing = 13, kRadius = 6;
