@@ -952,6 +952,10 @@ __device__ int find_slot(const int *first, int n, int t)
     return lo;
 }
 
+// The matrix-core layouts (a pass of 32 numbers costs the same however many of them are used).
+template <typename Sh> constexpr bool kCvMatrix = false;
+template <int NB> constexpr bool kCvMatrix<CvSharedM<NB>> = true;
+
 template <typename Sh>
 __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_h)
 {
@@ -1030,59 +1034,67 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_
         if (lane == 0) *seg_count = 0;
         return;
     }
-    const int ya = max(t.y0 - t.pad, 0), yb = min(t.y0 + t.rows + t.pad, t.h);
-    if (t.pairs) {
-        // An even width on a 4-byte aligned plane keeps every row's samples at even picture columns 4-byte aligned: a
-        // lane reads the two codes of one aligned pair in one load, window columns 2 lane - sh and 2 lane + 1 - sh
-        // (sh = 1 when the window starts at an odd column, and its column -1 is then left out).  A pair is inside or
-        // outside the picture as a whole (picture columns 2k, 2k + 1).
-        const int sh_ = (t.x0 - t.pad) & 1, cl = 2 * lane - sh_, xlo = t.x0 - t.pad + cl;
-        const unsigned xo = xlo >= 0 && xlo < t.w ? 2u * (unsigned)xlo : kCvOut;
-        const bool lo_ok = cl >= 0 && cl < cw, hi_ok = cl + 1 < cw;
-        for (int y = ya; y < yb; y += 32) {
-            unsigned v[32];
-#pragma unroll
-            for (int j = 0; j < 32; j++) {
-                const int yy = y + j < yb ? y + j : -1;
-                v[j] = __builtin_amdgcn_raw_buffer_load_b32(t.rs, xo + (unsigned)(yy * t.w * 2), 0, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < 32; j++) {
-                flag(occupied, lo_ok ? v[j] & 0xFFFFu : 0u);
-                flag(occupied, hi_ok ? v[j] >> 16 : 0u);
-            }
-        }
-    } else {
-        for (int y = ya; y < yb; y += 16) {
-            unsigned v[32];
-#pragma unroll
-            for (int j = 0; j < 16; j++) {
-                v[2 * j] = cv_at(t, c0 < cw ? cv_col(t, xl) : kCvOut, y + j < yb ? y + j : -1);
-                v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
-            }
-#pragma unroll
-            for (int j = 0; j < 32; j++) flag(occupied, v[j]);
-        }
-    }
-    __syncthreads();
-    // the relevant values, 64 per chunk, lane c holding chunk c: occupied and within 4 of a served value (the served
-    // set widened by 4 either way, across chunk edges), below nv; numbered in value order
+    __syncthreads();  // the served flags are in before they are read
+    // the served set, 64 values per chunk, lane c holding chunk c, and the values within 4 of a served one (the set
+    // widened by 4 either way, across chunk edges): all below nv
     const int nchunks = (nv + 63) / 64;
-    unsigned long long sc = 0, oc = 0;  // this lane's chunk of the two sets
+    unsigned long long sc = 0;  // this lane's chunk of the served set
     for (int c = 0; c < nchunks; c++) {
-        const int i = 64 * c + lane;
-        const unsigned long long sm = __builtin_amdgcn_ballot_w64(i < nv && served[i]);
-        const unsigned long long om = __builtin_amdgcn_ballot_w64(i < nv && occupied[i]);
-        if (lane == c) {
-            sc = sm;
-            oc = om;
-        }
+        const unsigned long long sm = __builtin_amdgcn_ballot_w64(64 * c + lane < nv && served[64 * c + lane]);
+        if (lane == c) sc = sm;
     }
     const unsigned long long sp = __shfl_up(sc, 1), sn = __shfl_down(sc, 1);
     unsigned long long near = sc;
 #pragma unroll
     for (int m = 1; m <= kDiffs; m++) near |= sc >> m | (lane + 1 < nchunks ? sn : 0ull) << (64 - m) | sc << m | (lane ? sp : 0ull) >> (64 - m);
-    const unsigned long long need = lane < nchunks ? oc & near : 0ull;
+    unsigned long long need = lane < nchunks ? near : 0ull;
+    // The relevant values: those within 4 of a served one that occur in the tile with its margins (numbered in value
+    // order).  When the matrix-core form takes all the values within 4 of a served one in one pass of 32, the scan of
+    // the margins is left out: a value that does not occur adds a number whose counts stay 0, and the tile takes one
+    // pass either way.
+    if (!kCvMatrix<Sh> || __builtin_amdgcn_readlane(wave_scan_incl((unsigned)__popcll(need)), 63) > 32) {
+        const int ya = max(t.y0 - t.pad, 0), yb = min(t.y0 + t.rows + t.pad, t.h);
+        if (t.pairs) {
+            // An even width on a 4-byte aligned plane keeps every row's samples at even picture columns 4-byte
+            // aligned: a lane reads the two codes of one aligned pair in one load, window columns 2 lane - sh and
+            // 2 lane + 1 - sh (sh = 1 when the window starts at an odd column, and its column -1 is then left out).  A
+            // pair is inside or outside the picture as a whole (picture columns 2k, 2k + 1).
+            const int sh_ = (t.x0 - t.pad) & 1, cl = 2 * lane - sh_, xlo = t.x0 - t.pad + cl;
+            const unsigned xo = xlo >= 0 && xlo < t.w ? 2u * (unsigned)xlo : kCvOut;
+            const bool lo_ok = cl >= 0 && cl < cw, hi_ok = cl + 1 < cw;
+            for (int y = ya; y < yb; y += 32) {
+                unsigned v[32];
+#pragma unroll
+                for (int j = 0; j < 32; j++) {
+                    const int yy = y + j < yb ? y + j : -1;
+                    v[j] = __builtin_amdgcn_raw_buffer_load_b32(t.rs, xo + (unsigned)(yy * t.w * 2), 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < 32; j++) {
+                    flag(occupied, lo_ok ? v[j] & 0xFFFFu : 0u);
+                    flag(occupied, hi_ok ? v[j] >> 16 : 0u);
+                }
+            }
+        } else {
+            for (int y = ya; y < yb; y += 16) {
+                unsigned v[32];
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    v[2 * j] = cv_at(t, c0 < cw ? cv_col(t, xl) : kCvOut, y + j < yb ? y + j : -1);
+                    v[2 * j + 1] = cv_at(t, c0 + 1 < cw ? cv_col(t, xl + 1) : kCvOut, y + j < yb ? y + j : -1);
+                }
+#pragma unroll
+                for (int j = 0; j < 32; j++) flag(occupied, v[j]);
+            }
+        }
+        __syncthreads();
+        unsigned long long oc = 0;  // this lane's chunk of the occupied set
+        for (int c = 0; c < nchunks; c++) {
+            const unsigned long long om = __builtin_amdgcn_ballot_w64(64 * c + lane < nv && occupied[64 * c + lane]);
+            if (lane == c) oc = om;
+        }
+        need &= oc;
+    }
     const unsigned count = (unsigned)__popcll(need), after = wave_scan_incl(count);
     const int relevant = (int)__builtin_amdgcn_readlane(after, 63);
     cv_number(sh, need, after - count, nchunks, nv);
