@@ -142,7 +142,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed steps before the timed ones")
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--batch", type=int, default=32, help="frame steps per step (with --tune 0)")
-    ap.add_argument("--tile-h", type=int, default=0)
+    ap.add_argument("--tile-h", type=int, default=0, help="c-value tile height (with --tune 0)")
+    ap.add_argument("--tile-h0", type=int, default=0, help="c-value tile height at full resolution, 0: --tile-h (with --tune 0)")
     ap.add_argument("--cv-bmax", type=int, default=0, help="c-value LDS layout, 32 or 64 (with --tune 0)")
     ap.add_argument("--cv-waves", type=int, default=0, help="persistent c-value waves per CU (with --tune 0)")
     ap.add_argument("--cv-tile-blocks", type=int, default=0, help="c-value kernel as one block per tile (with --tune 0)")
@@ -268,6 +269,7 @@ def main():
     # the autotuner, on the pool's first frames, outside the timed region; if it cannot run, the defaults
     tuned = None
     batch, tile_h, cv_bmax, cv_waves, cv_tb = args.batch, args.tile_h, args.cv_bmax, args.cv_waves, bool(args.cv_tile_blocks)
+    tile_h0 = args.tile_h0
     early, mfma, queues = bool(args.early_energies), bool(args.cv_mfma), args.cv_queues
     if args.tune:
         tune_descs = [es.EncodeDesc(d.w, d.h, pool, d.w, 2, 0) for d, _ in encs]
@@ -278,7 +280,7 @@ def main():
                                 serial_stages=bool(args.serial))
             batch, tile_h, cv_bmax, cv_waves = tuned["batch"], tuned["tile_h"], tuned["cv_bmax"], tuned["cv_waves"]
             cv_tb, early, mfma = tuned["cv_tile_blocks"], tuned["early_energies"], tuned["cv_mfma"]
-            queues = tuned.get("cv_queues", 0)
+            queues, tile_h0 = tuned.get("cv_queues", 0), tuned.get("tile_h0", 0)
         except es.EsError as exc:
             tuned = {"fallback": f"the defaults and the command line's settings: the autotuner failed ({exc})"}
         tuned["seconds"] = round(time.perf_counter() - t0, 2)
@@ -293,9 +295,9 @@ def main():
 
     # batched form (graph replay), with live per-stage events
     def batched(**kw):
-        return es.Scorer(es.ES_BATCHED, W, H, kw.pop("descs", descs), batch=batch, tile_h=tile_h, use_graph=True,
-                         src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves, cv_tile_blocks=cv_tb,
-                         early_energies=early, cv_mfma=mfma, cv_queues=queues, **kw)
+        return es.Scorer(es.ES_BATCHED, W, H, kw.pop("descs", descs), batch=batch, tile_h=tile_h, tile_h0=tile_h0,
+                         use_graph=True, src_bytes=sbytes, src_shift=sshift, cv_bmax=cv_bmax, cv_waves=cv_waves,
+                         cv_tile_blocks=cv_tb, early_energies=early, cv_mfma=mfma, cv_queues=queues, **kw)
 
     bat = batched(profile=True, serial_stages=bool(args.serial))
     submit(bat, 0, warm)
@@ -418,7 +420,8 @@ def main():
                                f"frames per file, reused in turn",
                    "config": args.config, "width": W, "height": H, "encodes": len(encs),
                    "step": f"one batch of {batch} frame steps (each the source's frame and every active encode's)",
-                   "frame_steps_per_step": batch, "tile_h": tile_h or 32, "cv_bmax": cv_bmax or 32,
+                   "frame_steps_per_step": batch, "tile_h": tile_h or 32, "tile_h0": tile_h0 or tile_h or 32,
+                   "cv_bmax": cv_bmax or 32,
                    "cv_waves": "one block per tile" if cv_tb else (cv_waves or 16),
                    "cv_method": "matrix cores" if mfma else "prefix scans",
                    "cv_queues": "one block per tile" if cv_tb else ("one per XCD, an eighth of every scale each"

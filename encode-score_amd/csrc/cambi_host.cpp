@@ -8,10 +8,11 @@ namespace es {
 static constexpr int kDefaultTileH = 32;
 static constexpr int kTopkBlocks = 64;
 
-CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues, int tile_h)
-    : info_(cambi_info(w, h, depth, window_base)), keep_(keep_cvalues), tile_h_(tile_h > 0 ? tile_h : kDefaultTileH)
+CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues, int tile_h, int tile_h0)
+    : info_(cambi_info(w, h, depth, window_base)), keep_(keep_cvalues)
 {
-    if (tile_h_ > kMaxTileH) throw Error(ES_EINVAL, "tile_h must be at most 4096");
+    if (tile_h > kMaxTileH || tile_h0 > kMaxTileH) throw Error(ES_EINVAL, "tile_h must be at most 4096");
+    cv_tile_heights(tile_h > 0 ? tile_h : kDefaultTileH, tile_h0, th_);
     ensure_init();
     const size_t total = (size_t)info_.off[kScales];
     CambiConsts k{};
@@ -30,7 +31,7 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     planes_.alloc(total * 2);
     if (keep_) cvals_.alloc(total * 4);
     int ntiles = 0;
-    const long long cap = cambi_list_layout(info_.sw, info_.sh, tile_h_, loff_, toff_, &ntiles);
+    const long long cap = cambi_list_layout(info_.sw, info_.sh, th_, loff_, toff_, &ntiles);
     list_.alloc((size_t)cap * 4);
     segs_.alloc((size_t)ntiles * 4);
     sums_.alloc(kScales * 8);
@@ -41,8 +42,8 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     for (int s = 0; s < kScales; s++) {
         jobs[s].list = list_.as<float>() + loff_[s];
         jobs[s].seg_count = segs_.as<unsigned>() + toff_[s];
-        jobs[s].nseg = cvalue_tiles(info_.sw[s], info_.sh[s], tile_h_);
-        jobs[s].seg_cap = kCvTileW * tile_h_;
+        jobs[s].nseg = cvalue_tiles(info_.sw[s], info_.sh[s], th_[s]);
+        jobs[s].seg_cap = kCvTileW * th_[s];
         jobs[s].count = counts_.as<unsigned>() + s;
         jobs[s].k = (unsigned long long)info_.k[s];
         jobs[s].hist = hist_.as<unsigned>() + (size_t)s * kTopkBlocks * 512;
@@ -54,7 +55,7 @@ CambiUnit::CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues
     int first[kScales + 1];
     first[0] = 0;
     for (int k = 0; k < kScales; k++)
-        first[k + 1] = first[k] + cvalue_tiles(info_.sw[kScales - 1 - k], info_.sh[kScales - 1 - k], tile_h_);
+        first[k + 1] = first[k] + cvalue_tiles(info_.sw[kScales - 1 - k], info_.sh[kScales - 1 - k], th_[kScales - 1 - k]);
     tiles_ = first[kScales];
     base_tiles_ = base_tiles(w, h);
     first_.alloc(sizeof first);
@@ -98,6 +99,7 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
         p->off[s] = info_.off[s];
         p->loff[s] = loff_[s];
         p->toff[s] = toff_[s];
+        p->th[s] = th_[s];
     }
     p->planes = planes_.as<uint16_t>();
     p->cvals = keep_ ? cvals_.as<float>() : nullptr;
@@ -116,7 +118,7 @@ void CambiUnit::launch(const void *img, int pitch, int bytes, int shift, hipStre
     if (keep_) hip_check(hipMemsetAsync(cvals_.p, 0, cvals_.bytes, own_), "clearing c-values");
     for (int s = 1; s < kScales; s++) launch_cambi_scale(dpic, 1, scale_strips(info_.sw[s], info_.sh[s]), s, own_);
     // one block per tile (window counts on the matrix cores unless set otherwise)
-    launch_cambi_cvalues(dpic, first_.as<int>(), kScales, tiles_, tile_h_, 0, nullptr, 0, bmax_, own_, mfma_ ? 1 : 0);
+    launch_cambi_cvalues(dpic, first_.as<int>(), kScales, tiles_, 0, nullptr, 0, bmax_, own_, mfma_ ? 1 : 0);
     for (int pass = 0; pass < 3; pass++) launch_topk_pass(jobs_.as<TopkJob>(), kScales, pass, kTopkBlocks, own_);
     launch_cambi_finish(dpic, jobs_.as<TopkJob>(), 1, score_.as<double>(), own_);
     Results *r = results_.as<Results>();

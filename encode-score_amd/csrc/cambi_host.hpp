@@ -7,7 +7,8 @@ namespace es {
 
 class CambiUnit {
 public:
-    CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues, int tile_h = 0);
+    // c-value tiles of tile_h rows (0: the default), tile_h0 at full resolution (0: tile_h)
+    CambiUnit(int w, int h, int depth, int window_base, bool keep_cvalues, int tile_h = 0, int tile_h0 = 0);
     ~CambiUnit();
     CambiUnit(const CambiUnit &) = delete;
     CambiUnit &operator=(const CambiUnit &) = delete;
@@ -32,7 +33,7 @@ private:
     };
     CambiInfo info_;
     bool keep_;
-    int tile_h_, tiles_ = 0, base_tiles_ = 0;
+    int th_[kScales], tiles_ = 0, base_tiles_ = 0;
     bool mfma_ = true;
     int bmax_ = 32;
     DevBuf consts_, planes_, cvals_, list_, segs_, sums_, counts_, jobs_, hist_, hsum_, pic_, first_, score_;

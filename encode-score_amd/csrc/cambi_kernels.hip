@@ -957,12 +957,13 @@ template <typename Sh> constexpr bool kCvMatrix = false;
 template <int NB> constexpr bool kCvMatrix<CvSharedM<NB>> = true;
 
 template <typename Sh>
-__device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local, int tile_h)
+__device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local)
 {
     const CambiConsts &K = *pic.consts;
     CvTile t;
     t.w = pic.sw[s];
     t.h = pic.sh[s];
+    const int tile_h = pic.th[s];
     t.rs = buffer_over(pic.planes + pic.off[s], (unsigned long long)t.w * t.h * 2);  // one tile per wave
     const int across = (t.w + kCvTileW - 1) / kCvTileW;
     t.x0 = (local % across) * kCvTileW;
@@ -1141,8 +1142,7 @@ __device__ int queue_tile(const int *tile_first, int np, unsigned k, unsigned v)
 // Without xcd_queues every wave takes tiles in order from counter[0].
 template <typename Sh>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCvWaves<Sh>))) void cambi_cvalues_kernel(
-    const CambiPic *pics, const int *tile_first, int nslots, int tile_h, int persistent, unsigned int *counter,
-    int xcd_queues)
+    const CambiPic *pics, const int *tile_first, int nslots, int persistent, unsigned int *counter, int xcd_queues)
 {
     __shared__ Sh sh;
     const int total = tile_first[nslots], np = nslots / kScales;
@@ -1167,21 +1167,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCvWaves<Sh>
         if (t >= total) break;
         const int slot = find_slot(tile_first, nslots, t);  // slots by scale, coarsest first, then by picture
         const int np = nslots / kScales, p = slot % np, s = kScales - 1 - slot / np;
-        cv_tile(sh, pics[p], s, t - tile_first[slot], tile_h);
+        cv_tile(sh, pics[p], s, t - tile_first[slot]);
         if (!persistent) break;
     }
 }
 
-void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma,
+void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int persistent,
+                          unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma,
                           int xcd_queues)
 {
     if (!persistent && total_tiles <= 0) return;
     const int grid = persistent ? blocks : total_tiles;
     auto k = mfma ? (bmax == 64 ? cambi_cvalues_kernel<CvSharedM<2>> : cambi_cvalues_kernel<CvSharedM<1>>)
                   : (bmax == 64 ? cambi_cvalues_kernel<CvShared<64>> : cambi_cvalues_kernel<CvShared<32>>);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, tile_h, persistent, work_counter,
-                       xcd_queues);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64), 0, st, pics, tile_first, nslots, persistent, work_counter, xcd_queues);
     hip_check(hipGetLastError(), "cambi c-values");
 }
 

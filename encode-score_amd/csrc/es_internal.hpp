@@ -57,13 +57,14 @@ __host__ __device__ constexpr unsigned plane_bias(int band_base) { return 0x8000
 
 // One picture for the CAMBI kernels.  The packed scale planes hold each pixel's code (above), scale s at
 // planes + off[s] (and the optional full c-value planes at cvals + off[s]).  The positive c-values of a scale go to
-// per-tile segments: tile j of scale s (64 columns by the launch's tile height) writes its values from
-// list + loff[s] + j * 64 * tile_h and their number to seg_count[toff[s] + j], so no tile waits on another.
+// per-tile segments: tile j of scale s (64 columns by th[s] rows) writes its values from list + loff[s] + j * 64 *
+// th[s] and their number to seg_count[toff[s] + j], so no tile waits on another.
 struct CambiPic {
     PlaneIn in;
     int sw[kScales], sh[kScales];
     long long off[kScales], loff[kScales];
     int toff[kScales];
+    int th[kScales];              // c-value tile height of each scale
     uint16_t *planes;
     float *cvals;                 // full c-value planes or nullptr
     float *list;                  // per-tile segments of positive c-values
@@ -86,17 +87,25 @@ struct TopkJob {
     unsigned long long above, above_fx;
 };
 
-// The per-tile segment layout of a picture's c-value lists for tiles of tile_h rows: list offsets and tile offsets
-// per scale; returns the capacity in values and the number of tiles.
-inline long long cambi_list_layout(const int *sw, const int *sh, int tile_h, long long *loff, int *toff, int *tiles)
+// The c-value tile height of each scale: tile_h0 rows at full resolution (0: tile_h), tile_h rows at the others.
+// Full-resolution tiles mostly take one pass, so taller ones spread the window's first rows and the tile's set-up
+// over more rows; the coarse scales' tiles take many passes each and stay short, so that there are enough of them.
+inline void cv_tile_heights(int tile_h, int tile_h0, int *th)
+{
+    for (int s = 0; s < kScales; s++) th[s] = s == 0 && tile_h0 > 0 ? tile_h0 : tile_h;
+}
+
+// The per-tile segment layout of a picture's c-value lists for tiles of th[s] rows at scale s: list offsets and tile
+// offsets per scale; returns the capacity in values and the number of tiles.
+inline long long cambi_list_layout(const int *sw, const int *sh, const int *th, long long *loff, int *toff, int *tiles)
 {
     long long cap = 0;
     int t = 0;
     for (int s = 0; s < kScales; s++) {
-        const int n = ((sw[s] + kCvTileW - 1) / kCvTileW) * ((sh[s] + tile_h - 1) / tile_h);
+        const int n = ((sw[s] + kCvTileW - 1) / kCvTileW) * ((sh[s] + th[s] - 1) / th[s]);
         loff[s] = cap;
         toff[s] = t;
-        cap += (long long)n * kCvTileW * tile_h;
+        cap += (long long)n * kCvTileW * th[s];
         t += n;
     }
     if (tiles) *tiles = t;
@@ -111,8 +120,8 @@ void launch_cambi_scale(const CambiPic *pics, int npics, int max_strips, int s, 
 // (nslots = npics * kScales: coarsest scale first); tile_first[nslots] holds the total; total_tiles sizes the grid
 // when not persistent.
 // mfma: window counts on the matrix cores instead of prefix scans (same results).
-void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int tile_h,
-                          int persistent, unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma,
+void launch_cambi_cvalues(const CambiPic *pics, const int *tile_first, int nslots, int total_tiles, int persistent,
+                          unsigned int *work_counter, int blocks, int bmax, hipStream_t st, int mfma,
                           int xcd_queues = 0);
 void launch_topk_pass(TopkJob *jobs, int njobs, int pass, int blocks, hipStream_t st);
 void launch_cambi_finish(const CambiPic *pics, TopkJob *jobs, int npics, double *scores, hipStream_t st);

@@ -250,7 +250,7 @@ public:
     {
         for (const Crop &cr : crops) {
             CropBufs b;
-            for (auto &u : b.cambi) u = std::make_unique<CambiUnit>(cr.w, cr.h, c.depth, window_base, false, c.tile_h);
+            for (auto &u : b.cambi) u = std::make_unique<CambiUnit>(cr.w, cr.h, c.depth, window_base, false, c.tile_h, c.tile_h0);
             b.ring.alloc((size_t)kRing * cr.w * cr.h * 4);
             hip_check(hipMemset(b.ring.p, 0, b.ring.bytes), "clearing");
             cb.push_back(std::move(b));
@@ -258,7 +258,7 @@ public:
         for (size_t e = 0; e < enc.size(); e++) {
             const Crop &cr = crops[enc[e].crop];
             EncBufs b;
-            for (auto &u : b.cambi) u = std::make_unique<CambiUnit>(cr.w, cr.h, c.depth, window_base, false, c.tile_h);
+            for (auto &u : b.cambi) u = std::make_unique<CambiUnit>(cr.w, cr.h, c.depth, window_base, false, c.tile_h, c.tile_h0);
             b.mid.alloc((size_t)(kRadius + 1) * cr.w * cr.h * 4);
             hip_check(hipMemset(b.mid.p, 0, b.mid.bytes), "clearing");
             for (int k = 0; k < 2; k++) {
@@ -443,8 +443,8 @@ public:
         B = c.batch > 0 ? c.batch : 16;
         require(B <= 64, "batch must be at most 64 frames");
         R = B + 16;
-        tile_h = c.tile_h > 0 ? c.tile_h : 32;
-        require(tile_h <= kMaxTileH, "tile_h must be at most 4096");
+        require(c.tile_h <= kMaxTileH && c.tile_h0 <= kMaxTileH, "tile_h must be at most 4096");
+        cv_tile_heights(c.tile_h > 0 ? c.tile_h : 32, c.tile_h0, th);
         int cus = 0;
         hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "device attributes");
         cv_blocks = std::max(1, cus) * (c.cv_waves > 0 ? c.cv_waves : 16);
@@ -481,7 +481,7 @@ public:
         lay.resize(C);
         for (int k = 0; k < C; k++) {
             const CambiInfo &ci = crops[k].ci;
-            lay[k].cap = cambi_list_layout(ci.sw, ci.sh, tile_h, lay[k].loff, lay[k].toff, &lay[k].tiles);
+            lay[k].cap = cambi_list_layout(ci.sw, ci.sh, th, lay[k].loff, lay[k].toff, &lay[k].tiles);
         }
         for (int p = 0; p < P; p++) {
             const int k = crop_of_role(p % roles);
@@ -505,8 +505,8 @@ public:
                 j = TopkJob{};
                 j.list = list.as<float>() + list_off[p] + lay[kc].loff[s];
                 j.seg_count = segs.as<unsigned>() + seg_off[p] + lay[kc].toff[s];
-                j.nseg = cvalue_tiles(ci.sw[s], ci.sh[s], tile_h);
-                j.seg_cap = kCvTileW * tile_h;
+                j.nseg = cvalue_tiles(ci.sw[s], ci.sh[s], th[s]);
+                j.seg_cap = kCvTileW * th[s];
                 j.count = counts.as<unsigned>() + (size_t)p * kScales + s;
                 j.k = (unsigned long long)ci.k[s];
                 j.hist = thist.as<unsigned>() + ((size_t)p * kScales + s) * kTopkBlocks * 512;
@@ -690,7 +690,7 @@ private:
         std::vector<double> cached;
     };
 
-    int B, R, P, roles, tile_h, cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
+    int B, R, P, roles, th[kScales], cv_bmax = 32, cv_blocks = 2048, max_base_tiles = 0, max_w = 0, max_h = 0, max_windows = 0;
     bool cv_tile_blocks = false, cv_mfma = false;
     bool cv_queues = false;  // persistent c-value waves take tiles from one queue per XCD (else one shared queue)
     std::vector<long long> max_px;
@@ -770,6 +770,7 @@ private:
                     pc.off[s] = ci.off[s];
                     pc.loff[s] = lay[c].loff[s];
                     pc.toff[s] = lay[c].toff[s];
+                    pc.th[s] = th[s];
                 }
                 if (on) {
                     pc.in = r < C ? src_plane(src[i], r) : enc_plane(encp[(size_t)i * E + (r - C)]);
@@ -793,7 +794,7 @@ private:
             for (int pidx = 0; pidx < P; pidx++) {
                 const int s = kScales - 1 - k;
                 first[k * P + pidx] = t;
-                t += cvalue_tiles(pics[pidx].sw[s], pics[pidx].sh[s], tile_h);
+                t += cvalue_tiles(pics[pidx].sw[s], pics[pidx].sh[s], th[s]);
             }
         first[P * kScales] = t;
         for (int i = 0; i < B; i++) {
@@ -891,7 +892,7 @@ private:
         case 3:
             // persistent waves taking tiles from a counter, or one block per tile of the round's largest layout (the
             // blocks past the round's tiles exit at once)
-            launch_cambi_cvalues(pics, dptr<int>(p, o_first()), P * kScales, (int)seg_off[P], tile_h, cv_tile_blocks ? 0 : 1,
+            launch_cambi_cvalues(pics, dptr<int>(p, o_first()), P * kScales, (int)seg_off[P], cv_tile_blocks ? 0 : 1,
                                  counts.as<unsigned>() + (size_t)P * kScales, cv_blocks, cv_bmax, st, cv_mfma ? 1 : 0,
                                  cv_queues ? 1 : 0);
             break;
@@ -1328,12 +1329,13 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         base.profile = 0;
         es_tune_result best{base.batch ? base.batch : 16, base.tile_h ? base.tile_h : 32, base.cv_bmax ? base.cv_bmax : 32,
                             base.cv_waves ? base.cv_waves : 16, 0.0, 0, base.cv_tile_blocks, base.early_energies,
-                            base.cv_mfma, base.cv_queues};
+                            base.cv_mfma, base.cv_queues, base.tile_h0};
         const int ne = base.n_encodes;
         auto trial = [&](const es_tune_result &t) -> double {
             es_scorer_config c = base;
             c.batch = t.batch;
             c.tile_h = t.tile_h;
+            c.tile_h0 = t.tile_h0;
             c.cv_bmax = t.cv_bmax;
             c.cv_waves = t.cv_waves;
             c.cv_tile_blocks = t.cv_tile_blocks;
@@ -1377,9 +1379,9 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
             for (int v : values) {
                 es_tune_result t = keep;
                 set(t, v);
-                if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.cv_bmax == keep.cv_bmax && t.cv_waves == keep.cv_waves &&
-                    t.cv_tile_blocks == keep.cv_tile_blocks && t.early_energies == keep.early_energies &&
-                    t.cv_mfma == keep.cv_mfma && t.cv_queues == keep.cv_queues)
+                if (t.batch == keep.batch && t.tile_h == keep.tile_h && t.tile_h0 == keep.tile_h0 && t.cv_bmax == keep.cv_bmax &&
+                    t.cv_waves == keep.cv_waves && t.cv_tile_blocks == keep.cv_tile_blocks &&
+                    t.early_energies == keep.early_energies && t.cv_mfma == keep.cv_mfma && t.cv_queues == keep.cv_queues)
                     continue;
                 const double ms = trial(t);
                 if (ms > 0 && ms < best.ms_per_step) {
@@ -1393,6 +1395,8 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         // tile height and LDS layout together: taller tiles see more values, which favours the wider passes
         try_set([](es_tune_result &t, int v) { t.tile_h = v % 1000, t.cv_bmax = v / 1000; },
                 {32016, 32024, 32032, 32048, 32064, 32096, 32128, 64016, 64024, 64032, 64048, 64064, 64096, 64128});
+        // full-resolution tiles, mostly one pass each, taller than the others (0: the same height)
+        try_set([](es_tune_result &t, int v) { t.tile_h0 = v; }, {0, 64, 96, 128, 192});
         // with the energy chain overlapping, fewer resident c-value waves can leave it room
         // (0: one block per tile, which the hardware interleaves with the overlapping energy chain)
         try_set([](es_tune_result &t, int v) { t.cv_waves = v, t.cv_tile_blocks = v == 0; }, {0, 6, 8, 12, 16, 24});

@@ -48,12 +48,13 @@ class ScorerConfig(C.Structure):
                 ("depth", _i), ("win", _i), ("per_segment", _i), ("window_base", _i), ("batch", _i), ("tile_h", _i),
                 ("use_graph", _i), ("n_encodes", _i), ("encodes", C.POINTER(EncodeDesc)), ("profile", _i),
                 ("cv_bmax", _i), ("cv_waves", _i), ("serial_stages", _i), ("cv_tile_blocks", _i), ("early_energies", _i),
-                ("cv_mfma", _i), ("cv_queues", _i)]
+                ("cv_mfma", _i), ("cv_queues", _i), ("tile_h0", _i)]
 
 
 class TuneResult(C.Structure):
     _fields_ = [("batch", _i), ("tile_h", _i), ("cv_bmax", _i), ("cv_waves", _i), ("ms_per_step", C.c_double),
-                ("trials", _i), ("cv_tile_blocks", _i), ("early_energies", _i), ("cv_mfma", _i), ("cv_queues", _i)]
+                ("trials", _i), ("cv_tile_blocks", _i), ("early_energies", _i), ("cv_mfma", _i), ("cv_queues", _i),
+                ("tile_h0", _i)]
 
 
 class Segment(C.Structure):
@@ -389,12 +390,12 @@ class Scorer:
     def __init__(self, mode, src_w, src_h, encodes, depth=10, win=16, per_segment=1440, window_base=65, batch=16,
                  tile_h=0, use_graph=False, src_pitch=None, src_bytes=2, src_shift=0, profile=False, cv_bmax=0,
                  cv_waves=0, serial_stages=False, cv_tile_blocks=False, early_energies=False, cv_mfma=False,
-                 cv_queues=0):
+                 cv_queues=0, tile_h0=0):
         self._encodes = (EncodeDesc * len(encodes))(*encodes)
         cfg = ScorerConfig(mode, src_w, src_h, src_pitch or src_w, src_bytes, src_shift, depth, win, per_segment,
                            window_base, batch, tile_h, int(use_graph), len(encodes), self._encodes, int(profile),
                            cv_bmax, cv_waves, int(serial_stages), int(cv_tile_blocks), int(early_energies),
-                           int(cv_mfma), int(cv_queues))
+                           int(cv_mfma), int(cv_queues), tile_h0)
         p = _vp()
         check(lib.es_scorer_create(C.byref(cfg), C.byref(p)))
         self.ptr = p.value
@@ -590,8 +591,9 @@ def synth_frame_host(desc, frame, bytes=2, threads=8):
 def autotune(src_w, src_h, encodes, src_ptrs, enc_ptrs, depth=10, win=16, src_bytes=2, src_shift=0, cache=None,
              serial_stages=False):
     """The library's autotuner (es_scorer_autotune) on the given device frames (len(src_ptrs) steps; enc_ptrs[i] holds
-    step i's encode planes): a dict of batch, tile_h, cv_bmax, cv_waves, ms_per_step and trials.  With `cache` (a
-    JSON file path) a result is stored under the workload's shape and reused."""
+    step i's encode planes): a dict of batch, tile_h, tile_h0, cv_bmax, cv_waves, the c-value method, queues and grid,
+    when the energies start, ms_per_step and trials.  With `cache` (a JSON file path) a result is stored under the
+    workload's shape and reused."""
     import json
     key = json.dumps({"src": [src_w, src_h, src_bytes, src_shift], "enc": [[e.w, e.h, e.bytes, e.shift] for e in encodes],
                       "depth": depth, "win": win, "serial": bool(serial_stages), "device": _device_key()}, sort_keys=True)
@@ -610,7 +612,7 @@ def autotune(src_w, src_h, encodes, src_ptrs, enc_ptrs, depth=10, win=16, src_by
     enc = (_vp * len(flat))(*flat)
     r = TuneResult()
     check(lib.es_scorer_autotune(C.byref(cfg), n, src, enc, C.byref(r)))
-    out = {"batch": r.batch, "tile_h": r.tile_h, "cv_bmax": r.cv_bmax, "cv_waves": r.cv_waves,
+    out = {"batch": r.batch, "tile_h": r.tile_h, "tile_h0": r.tile_h0, "cv_bmax": r.cv_bmax, "cv_waves": r.cv_waves,
            "cv_tile_blocks": bool(r.cv_tile_blocks), "early_energies": bool(r.early_energies),
            "cv_mfma": bool(r.cv_mfma), "cv_queues": r.cv_queues,
            "ms_per_step": r.ms_per_step, "trials": r.trials}

@@ -268,6 +268,7 @@ typedef struct {
     int cv_queues;      /* batched form, persistent c-value waves: one tile queue shared by every XCD (0), or one
                          * queue per XCD, each with an eighth of every scale's tiles, so that tiles sharing halo rows
                          * and columns meet in one L2 (1) */
+    int tile_h0;        /* c-value tile height at full resolution (scale 0), 0 = tile_h; at most 4096 */
 } es_scorer_config;
 
 /* One segment of one encode (Accum, src/stats.hpp:36-82), with its histograms' quantiles and the score parts.
@@ -299,12 +300,14 @@ typedef struct {
     double ms_per_step;
     int trials;
     int cv_tile_blocks, early_energies, cv_mfma, cv_queues;
+    int tile_h0;
 } es_tune_result;
 /* Autotuner (no counterpart in the reference): times the batched form of cfg's workload on the caller's frames
  * (nframes steps as for es_scorer_submit, at least three batches of the largest batch tried), trying the c-value
- * c-value method (matrix cores or prefix scans), the tile height with the c-value LDS layout (together), then the
- * c-value waves per CU or one block per tile, then the persistent waves' queues, then when the energy chain starts,
- * then the batch, and returns the fastest.  Each trial allocates a scorer of its own. */
+ * method (matrix cores or prefix scans), the tile height with the c-value LDS layout (together), then the tile height
+ * at full resolution, then the c-value waves per CU or one block per tile, then the persistent waves' queues, then
+ * when the energy chain starts, then the batch, and returns the fastest.  Each trial allocates a scorer of its
+ * own. */
 int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *const *src, const void *const *enc,
                        es_tune_result *out);
 int es_scorer_destroy(es_scorer *s);

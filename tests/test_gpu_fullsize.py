@@ -108,9 +108,9 @@ def test_c1_ramp_1080p(gpu, orc):
 
 
 def test_c2_mosaic_4k(gpu, orc):
-    """C2 at 3840x2160: matrix cores at the autotuner's tile height and prefix scans with the wide layout."""
+    """C2 at 3840x2160: matrix cores at the autotuner's tile heights and prefix scans with the wide layout."""
     scorer_vs_oracle(gpu, orc, 2, 14, [0],
-                     settings=(dict(cv_mfma=True, tile_h=64, cv_bmax=32, batch=8),
+                     settings=(dict(cv_mfma=True, tile_h=64, tile_h0=128, cv_bmax=32, batch=8),
                                dict(cv_mfma=False, tile_h=32, cv_bmax=64, batch=8, cv_waves=12),
                                dict(cv_mfma=True, tile_h=128, cv_bmax=32, batch=8, cv_queues=1)))
 
@@ -124,9 +124,9 @@ def test_c3_uneven_crops(gpu, orc):
 
 def test_c4_8bit_source_three_encodes(gpu, orc):
     """C4: an 8-bit source read as 1-byte samples shifted by 2 against three 10-bit encodes, graph replay."""
-    scorer_vs_oracle(gpu, orc, 4, 14, [0, 1, 2], settings=(dict(cv_mfma=True, tile_h=64, batch=8),))
+    scorer_vs_oracle(gpu, orc, 4, 14, [0, 1, 2], settings=(dict(cv_mfma=True, tile_h=64, tile_h0=96, batch=8),))
 
 
 def test_c5_3840x1600_multi_encode(gpu, orc):
     """C5 at 3840x1600 (window 59): three of its eight encodes with their blur, quantisation and grain mix."""
-    scorer_vs_oracle(gpu, orc, 5, 14, [0, 3, 7], settings=(dict(cv_mfma=True, tile_h=64, batch=8),))
+    scorer_vs_oracle(gpu, orc, 5, 14, [0, 3, 7], settings=(dict(cv_mfma=True, tile_h=64, tile_h0=128, batch=8),))

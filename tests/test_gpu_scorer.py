@@ -180,20 +180,21 @@ def test_batched_larger_windows_match_oracle(gpu, orc, win):
 
 
 @pytest.mark.parametrize("mfma", [False, True])
-@pytest.mark.parametrize("bmax,tile_h,waves", [(64, 32, 16), (64, 16, 8), (32, 48, 24), (32, 64, 12), (32, 32, 0),
-                                               (64, 24, 0), (32, 360, 16)])
-def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, waves, mfma):
-    """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, waves per CU or
-    one block per tile (waves 0)) gives the oracle's results; textured 640x360 frames have tiles with more than 32
-    relevant values."""
+@pytest.mark.parametrize("bmax,tile_h,tile_h0,waves", [(64, 32, 0, 16), (64, 16, 48, 8), (32, 48, 0, 24),
+                                                       (32, 64, 128, 12), (32, 32, 96, 0), (64, 24, 0, 0),
+                                                       (32, 360, 0, 16), (32, 64, 192, 16)])
+def test_cvalue_layouts_and_tiles_match_oracle(gpu, orc, bmax, tile_h, tile_h0, waves, mfma):
+    """Every setting the autotuner may pick (c-value passes of up to 32 or 64 numbers, tile heights, a separate tile
+    height at full resolution, waves per CU or one block per tile (waves 0)) gives the oracle's results; textured
+    640x360 frames have tiles with more than 32 relevant values."""
     n, w, h = 14, 640, 360
-    print(f"case layout bmax {bmax} tile_h {tile_h} waves {waves} mfma {mfma}", flush=True)
+    print(f"case layout bmax {bmax} tile_h {tile_h} tile_h0 {tile_h0} waves {waves} mfma {mfma}", flush=True)
     src, enc = frames_of(gpu, n, w=w, h=h, seed=41, full=(w, h))
     run, cam, cam_src = oracle_run(orc, src, enc, 1440)
     ds = [gpu.DeviceArray.from_host(f) for f in src]
     de = [gpu.DeviceArray.from_host(f) for f in enc]
     sc = gpu.Scorer(gpu.ES_BATCHED, w, h, [gpu.EncodeDesc(w, h, n, w, 2, 0)], batch=8, use_graph=True, tile_h=tile_h,
-                    cv_bmax=bmax, cv_waves=waves, cv_tile_blocks=waves == 0, cv_mfma=mfma)
+                    tile_h0=tile_h0, cv_bmax=bmax, cv_waves=waves, cv_tile_blocks=waves == 0, cv_mfma=mfma)
     sc.submit([d.ptr for d in ds], [[d.ptr] for d in de])
     sc.sync()
     a, b = sc.cambi(0, n)
@@ -208,6 +209,8 @@ def test_tile_height_is_bounded(gpu):
     # a lane sums a tile's c-values exactly in FP64 for tiles of at most 4096 rows
     with pytest.raises(gpu.EsError, match="tile_h"):
         gpu.Scorer(gpu.ES_BATCHED, 64, 64, [gpu.EncodeDesc(64, 64, 4, 64, 2, 0)], batch=4, tile_h=4097)
+    with pytest.raises(gpu.EsError, match="tile_h"):
+        gpu.Scorer(gpu.ES_BATCHED, 64, 64, [gpu.EncodeDesc(64, 64, 4, 64, 2, 0)], batch=4, tile_h0=4097)
 
 
 def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
@@ -218,13 +221,14 @@ def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
     descs = [gpu.EncodeDesc(W, H, n, W, 2, 0)]
     cache = str(tmp_path / "tune.json")
     r = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
-    assert r["batch"] in (8, 16, 32) and r["tile_h"] in (16, 24, 32, 48, 64)
+    assert r["batch"] in (8, 16, 32) and r["tile_h"] in (16, 24, 32, 48, 64, 96, 128)
+    assert r["tile_h0"] in (0, 64, 96, 128, 192)
     assert r["cv_bmax"] in (32, 64) and r["cv_waves"] in (0, 6, 8, 12, 16, 24)
     assert r["cv_tile_blocks"] == (r["cv_waves"] == 0) and r["early_energies"] in (False, True)
     assert r["ms_per_step"] > 0 and r["trials"] >= 14 and not r["cached"]
     again = gpu.autotune(W, H, descs, [d.ptr for d in ds], [[d.ptr] for d in de], cache=cache)
-    assert again["cached"] and {k: again[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")} == \
-        {k: r[k] for k in ("batch", "tile_h", "cv_bmax", "cv_waves")}
+    assert again["cached"] and {k: again[k] for k in ("batch", "tile_h", "tile_h0", "cv_bmax", "cv_waves")} == \
+        {k: r[k] for k in ("batch", "tile_h", "tile_h0", "cv_bmax", "cv_waves")}
     with pytest.raises(gpu.EsError):  # too few frames for three batches
         gpu.autotune(W, H, descs, [d.ptr for d in ds[:10]], [[d.ptr] for d in de[:10]])
 
