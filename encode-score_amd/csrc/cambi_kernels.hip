@@ -657,16 +657,21 @@ __device__ __forceinline__ cv_v4i band_bytes(int xr, int cb, int span)
 }
 
 // Passes of NBU blocks of 32 numbers (NBU <= NB, the blocks the LDS layout holds); the band of every output block
-// touches NJ margin blocks.  Per pass each value's row in the tables is set once (row[], doubled): its number less
+// touches NJ margin blocks.  k16: one pass of 16 numbers, for a tile with at most 16 relevant values: the products are
+// v_mfma_i32_16x16x64_i8 over output blocks of 16 columns and margin blocks of 64 (half the operand reads and product
+// stores of a 32-number pass), and the window table is laid out as rows of 18 entries per output column.  Per pass each value's row in the tables is set once (row[], doubled): its number less
 // b0, or B when it is outside the pass or not relevant.  Window counts of such values read the zero entry B of the
 // pixel's column, and their count updates go to row B of cnt, which nothing reads.  A sample's index is its code
 // clamped to nv + 1 (one instruction): value - band_base + 5 for a masked sample in the band, and every index outside
 // 5 .. nv - 4 has row B (0 for a sample outside the picture).
-template <int NB, int NBU, int NJ, bool kPairs>
+template <int NB, int NBU, int NJ, bool kPairs, bool k16>
 __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c, unsigned &tcount)
 {
-    static_assert(NBU >= 1 && NBU <= NB && NJ >= 1 && NJ <= 3, "blocks of numbers, band blocks");
-    constexpr int B = 32 * NBU;
+    static_assert(NBU >= 1 && NBU <= NB && NJ >= 1 && NJ <= 3 && (!k16 || NBU == 1), "blocks of numbers, band blocks");
+    constexpr int B = k16 ? 16 : 32 * NBU;
+    // k16: u16 entries per output column in the window table (16 numbers, a zero at 16 and one of spread: 9 words, an
+    // odd stride) and the words between output blocks of 16 columns
+    constexpr int kW16 = 18, kOb16 = 16 * kW16 / 2;
     const int lane = threadIdx.x, pad = t.pad, span = 2 * pad, cwid = kCvTileW + 2 * pad;
     // this lane's two margin columns c0 and c0 + 1: 2 lane and 2 lane + 1, or with kPairs the 4-byte aligned pair of
     // picture columns, 2 lane - sh and 2 lane + 1 - sh (sh = 1 when the window starts at an odd column), read as one
@@ -679,13 +684,21 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
     // kPairs: the pair is inside or outside the picture as a whole (picture columns 2k, 2k + 1)
     const unsigned xop = xl >= 0 && xl < t.w ? 2u * (unsigned)xl : kCvOut;
     // band operands of output block mb and margin block mb + j: this lane's output column 32 mb + lane % 32 and its
-    // half of the block's margin columns
-    const int g = lane >> 5, r32 = lane & 31;
-    cv_v4i band[2][NJ];
+    // half of the block's margin columns; k16: of output block ob and margin block kb, output column 16 ob + lane % 16
+    // and its quarter of the block's margin columns 64 kb ..
+    const int g = lane >> 5, r32 = lane & 31, g4 = lane >> 4, r16 = lane & 15;
+    cv_v4i band[k16 ? 4 : 2][k16 ? 2 : NJ];
+    if constexpr (k16) {
 #pragma unroll
-    for (int mb = 0; mb < 2; mb++)
+        for (int ob = 0; ob < 4; ob++)
 #pragma unroll
-        for (int j = 0; j < NJ; j++) band[mb][j] = band_bytes(32 * mb + r32, 32 * (mb + j) + 16 * g, span);
+            for (int kb = 0; kb < 2; kb++) band[ob][kb] = band_bytes(16 * ob + r16, 64 * kb + 16 * g4, span);
+    } else {
+#pragma unroll
+        for (int mb = 0; mb < 2; mb++)
+#pragma unroll
+            for (int j = 0; j < NJ; j++) band[mb][j] = band_bytes(32 * mb + r32, 32 * (mb + j) + 16 * g, span);
+    }
     // the count words of the lane's two margin columns in every number's row (the same word unless sh = 1): their
     // byte offsets in the row, and the increments of the columns' bytes (a decrement is the negated increment; 0 for
     // a column outside the window)
@@ -706,23 +719,28 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
             b = cv_at(t, xo1, y);
         }
     };
-    // this lane's start in the count table (A operand: number r32, its 16 margin columns of a block)
-    const uint8_t *const crow = &sh.cnt[r32][16 * g];
-    // the window table's words of the products this lane writes (output column 32 mb + r32, numbers 4 g ..), as one
-    // LDS address per output block held in a register, so that every store is that address plus an immediate offset
+    // this lane's start in the count table (A operand: number r32, its 16 margin columns of a block; k16: number r16,
+    // its 16 margin columns of a block of 64)
+    const uint8_t *const crow = k16 ? &sh.cnt[r16][16 * g4] : &sh.cnt[r32][16 * g];
+    // the window table's words of the products this lane writes (output column 32 mb + r32, numbers 4 g ..; k16:
+    // output columns 16 ob + r16 for ob 0, 1 and 2, 3, numbers 4 g4 ..), as one LDS address per output block (k16: per
+    // pair of them) held in a register, so that every store is that address plus an immediate offset
     typedef __attribute__((address_space(3))) uint32_t lds_word;
-    lds_word *wcol[2] = {(lds_word *)&sh.win[r32][4 * g], (lds_word *)&sh.win[32 + r32][4 * g]};
+    uint16_t *const w16 = &sh.win[0][0];  // k16: the table as rows of kW16 entries
+    lds_word *wcol[2] = {(lds_word *)(k16 ? w16 + r16 * kW16 + 4 * g4 : &sh.win[r32][4 * g]),
+                         (lds_word *)(k16 ? w16 + (32 + r16) * kW16 + 4 * g4 : &sh.win[32 + r32][4 * g])};
     asm volatile("" : "+v"(wcol[0]), "+v"(wcol[1]));
     // the row of this lane's own output column, read by its served pixel at byte offset 2 n for number n
     typedef __attribute__((address_space(3))) const uint8_t lds_byte;
-    lds_byte *const wmine = (lds_byte *)&sh.win[lane][0];
+    lds_byte *const wmine = (lds_byte *)(k16 ? w16 + lane * kW16 : &sh.win[lane][0]);
     const unsigned nv = (unsigned)(t.band_size + 2 * kDiffs);
     auto index = [&](unsigned v) { return min(v, nv + 1); };
     const uint8_t *const rows = &sh.row[kDiffs];  // rows[i]: the doubled row of index i (i from -4 to nv + 5)
     for (int b0 = 0;;) {
         const int lo = b0 ? b0 + kDiffs : 0, hi = b0 + B >= relevant ? relevant : b0 + B - kDiffs;
         for (int i = lane; i < B * kMCnt / 16; i += 64) ((cv_v4i *)cnt)[i] = cv_v4i{0, 0, 0, 0};
-        ((uint32_t *)&sh.win[lane][B])[0] = 0u;  // the zero entry B of every column (the setup's flags lived here)
+        // the zero entry B of every column (the setup's flags lived here)
+        ((uint32_t *)(k16 ? w16 + lane * kW16 + B : &sh.win[lane][B]))[0] = 0u;
         // the rows of the pass: a relevant value's number is the count of relevant values below it
         for (int j = 0, below = 0; j * 64 < (int)nv; j++) {
             const unsigned long long m = sh.relevant[j];
@@ -781,6 +799,31 @@ __device__ void cv_passes_m(CvSharedM<NB> &sh, const CvTile &t, int relevant, do
                 // 32 .. 63), the two blocks' chains interleaved so that neither waits on its own previous product
                 auto products = [&](auto U0, auto U1) __attribute__((always_inline)) {
                     constexpr bool use[2] = {decltype(U0)::value, decltype(U1)::value};
+                    if constexpr (k16) {
+                        // output blocks 0, 1 are the left half, 2, 3 the right; each the sum over two margin blocks
+                        cv_v4i aop[2], acc[4];
+#pragma unroll
+                        for (int kb = 0; kb < 2; kb++) aop[kb] = *(const cv_v4i *)(crow + 64 * kb);
+#pragma unroll
+                        for (int ob = 0; ob < 4; ob++)
+                            if (use[ob / 2]) acc[ob] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop[0], band[ob][0], cv_v4i{}, 0, 0, 0);
+#pragma unroll
+                        for (int ob = 0; ob < 4; ob++)
+                            if (use[ob / 2]) acc[ob] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop[1], band[ob][1], acc[ob], 0, 0, 0);
+                        const uint32_t *const rw = (const uint32_t *)&sh.row[io & ~3u];
+                        d0 = rw[0];
+                        d1 = rw[1];
+                        d2 = rw[2];
+                        // D's register i of this lane: number 4 (lane / 16) + i, output column 16 ob + lane % 16; two
+                        // numbers per 4-byte store
+#pragma unroll
+                        for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+                            for (int e = 0; e < 2; e++)
+                                if (use[ob / 2])
+                                    wcol[ob / 2][(ob & 1) * kOb16 + e] = (unsigned)acc[ob][2 * e] | (unsigned)acc[ob][2 * e + 1] << 16;
+                        return;
+                    }
                     constexpr int kb0 = use[0] ? 0 : 1, kb1 = use[1] ? NJ : NJ - 1;
                     cv_v4i aop[NBU][NJ + 1];
 #pragma unroll
@@ -902,17 +945,23 @@ __device__ __forceinline__ void cv_run_m(CvSharedM<NB> &sh, const CvTile &t, int
                                          unsigned &tcount)
 {
     if (t.pairs) {
-        if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ, true>(sh, t, relevant, sum_c, tcount);
-        else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ, true>(sh, t, relevant, sum_c, tcount);
+        if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ, true, false>(sh, t, relevant, sum_c, tcount);
+        else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ, true, false>(sh, t, relevant, sum_c, tcount);
     } else {
-        if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ, false>(sh, t, relevant, sum_c, tcount);
-        else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ, false>(sh, t, relevant, sum_c, tcount);
+        if (NB == 1 || relevant <= 32) cv_passes_m<NB, 1, NJ, false, false>(sh, t, relevant, sum_c, tcount);
+        else cv_passes_m<NB, (NB > 1 ? 2 : 1), NJ, false, false>(sh, t, relevant, sum_c, tcount);
     }
 }
 template <int NB>
 __device__ __forceinline__ void cv_run(CvSharedM<NB> &sh, const CvTile &t, int relevant, double &sum_c,
                                        unsigned &tcount)
 {
+    // a tile with at most 16 relevant values takes one pass of 16 numbers
+    if (relevant <= 16) {
+        if (t.pairs) cv_passes_m<NB, 1, 1, true, true>(sh, t, relevant, sum_c, tcount);
+        else cv_passes_m<NB, 1, 1, false, true>(sh, t, relevant, sum_c, tcount);
+        return;
+    }
     // the margin blocks the band of an output block touches: 2 pad + 1 columns from the block's first
     const int nj = min((31 + 2 * t.pad) / 32 + 1, 3);
     if (nj == 3) cv_run_m<NB, 3>(sh, t, relevant, sum_c, tcount);
