@@ -81,8 +81,8 @@ STAGE_KERNEL = {"cambi_cvalues": "void cambi_cvalues_kernel<{layout}>", "cambi_b
 # summarised per dispatch; see each file's "command"
 PMC_FILES = {(1, 1920, 1080): "pmc_1920x1080_ramp.json", (2, 3840, 2160): "pmc_3840x2160_mosaic.json",
              (3, 1920, 1080): "pmc_1920x1080_crops_4enc.json", (4, 1920, 1080): "pmc_1920x1080_8bit_3enc.json"}
-PMC_MISSING = {5: "no PMC summary: rocprofv3's WRITE_SIZE pass over this workload stopped printing while the run "
-                  "generated its frames and was killed; its FETCH_SIZE pass alone is in "
+PMC_MISSING = {5: "no PMC summary: rocprofv3's FETCH_SIZE and WRITE_SIZE passes over this workload stopped printing "
+                  "while the run generated its frames and were killed; the kernel trace alone is in "
                   "profiles/measure_path_3840x1600_mosaic_8enc.md"}
 
 
