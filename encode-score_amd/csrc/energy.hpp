@@ -6,12 +6,15 @@ namespace es {
 
 // One (source, encode) pair for window_stats: both planes cropped to w x h (the source's pointer already offset to
 // its crop), luma brought to the 10-bit scale by `up`.  mid_r / mid_d receive M = 81 mid on the area whole windows
-// cover (either may be null); sums receives 12 exact sums per window.
+// cover (either may be null); sums receives 12 exact sums per window.  sse (may be null) += the luma squared error of
+// the pair over that same area at the pipeline depth (the samples before `up`), the share of the frame-sums job whose
+// lw x lh area it is.
 struct WsPair {
     PlaneIn r, d;
     int w, h, up;
     int32_t *mid_r, *mid_d;
     long long *sums;
+    unsigned long long *sse;
     // FP64 form only (any may be null): mid = M / 81 as float planes, and the reference's 8 figures per window as
     // floats (src/kernels.cu:340-347)
     float *midf_r, *midf_d;

@@ -262,6 +262,10 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
     }
     int accF[3] = {0, 0, 0}, accG[6] = {0, 0, 0, 0, 0, 0};
     long long accM[3] = {0, 0, 0};
+    // the luma squared error of the lane's column over the strip's rows: at most 64 rows of d^2 with d <= 1023 at the
+    // 10-bit scale, below 2^26; lanes right of the covered width add zero
+    unsigned sse = 0;
+    const unsigned keep = inside && P.sse ? ~0u : 0u;
     long long winF[3] = {0, 0, 0}, winG[6] = {0, 0, 0, 0, 0, 0}, winM[3] = {0, 0, 0};  // WIN > 16 only
     // a closed 16-row block of one group: into the lane's window sums, and when the block closes a row of windows, the
     // window sums: each lane leaves its N sums in LDS, then lane k of each window's WIN lanes adds up sum k of them
@@ -303,6 +307,10 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
             for (int p = 0; p < 2; p++) {
                 rowbuf[buf][p][lane + 4] = pr[i % 3][p];
                 rowbuf[buf][p][hslot] = ph[i % 3][p];
+            }
+            if (r >= y0 && r < y1) {  // uniform
+                const unsigned d = __usad((unsigned)pr[i % 3][0], (unsigned)pr[i % 3][1], 0u) & keep;
+                sse = __umul24(d, d) + sse;
             }
 #pragma unroll
             for (int p = 0; p < 2; p++) {
@@ -363,6 +371,10 @@ __device__ __forceinline__ void ws_strip(const WsPair &P, WsRows &rowbuf, WsRed 
                 if (ym % 16 == 15) flush(accM, winM, 1, 2, ym);
             }
         }
+    }
+    if (P.sse) {  // uniform
+        const long long t = wave_total64((long long)sse);
+        if (lane == 0) atomicAdd(P.sse, (unsigned long long)t >> (2 * P.up));
     }
 }
 

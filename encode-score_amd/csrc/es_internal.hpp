@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -87,12 +88,16 @@ struct TopkJob {
     unsigned long long above, above_fx;
 };
 
-// The c-value tile height of each scale: tile_h0 rows at full resolution (0: tile_h), tile_h rows at the others.
-// Full-resolution tiles mostly take one pass, so taller ones spread the window's first rows and the tile's set-up
-// over more rows; the coarse scales' tiles take many passes each and stay short, so that there are enough of them.
+// The c-value tile height of each scale: tile_h0 rows at full resolution (0: tile_h), tile_h rows at scale 1 and at
+// most kCoarseTileH rows at scales 2-4.  Full-resolution tiles mostly take one pass, so taller ones spread the
+// window's first rows and the tile's set-up over more rows; a coarse scale's tile spans many more distinct values and
+// takes many passes (10-13 on a 4K mosaic at scale 4), so its tiles stay short: fewer values each, and enough tiles
+// to spread over the GPU.
+constexpr int kCoarseTileH = 32;
 inline void cv_tile_heights(int tile_h, int tile_h0, int *th)
 {
-    for (int s = 0; s < kScales; s++) th[s] = s == 0 && tile_h0 > 0 ? tile_h0 : tile_h;
+    for (int s = 0; s < kScales; s++)
+        th[s] = s == 0 ? (tile_h0 > 0 ? tile_h0 : tile_h) : s == 1 ? tile_h : std::min(tile_h, kCoarseTileH);
 }
 
 // The per-tile segment layout of a picture's c-value lists for tiles of th[s] rows at scale s: list offsets and tile

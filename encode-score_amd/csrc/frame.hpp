@@ -15,11 +15,14 @@ struct ChromaIn {
 // The figures of one (source, encode) frame pair beside the measures (src/main.cpp:543-556): both frames cropped to
 // w x h (w, h even) at the pipeline depth.  out[0..2] += the luma, u and v squared errors; out[3..5] += the
 // encode's colour sums (10-bit u, v and 2^23 times the strength, see strength_fixed); with src_color, out[6..8] +=
-// the source's.  The sums are integers, so the order the blocks add them in does not change them.
+// the source's.  The luma area [0, lw) x [0, lh) is left out of out[0]: the window-stats kernel adds its squared
+// error there (lw = lh = 0: none; lh a multiple of kFrameSumRows).  The sums are integers, so the order the blocks
+// and kernels add them in does not change them.
 struct FrameSumJob {
     PlaneIn ry, dy;
     ChromaIn ru, rv, du, dv;
     int w, h, up, src_color;
+    int lw, lh;
     unsigned long long *out;
 };
 constexpr int kFrameSums = 9;

@@ -229,7 +229,8 @@ void launch_color_to_double(const unsigned long long *exact, double *sums, hipSt
 }
 
 // ---------------------------------------------------------------------------------------------------------------
-// frame_sums: per block kFrameSumRows luma rows and half as many chroma rows of one job.  A lane takes kSumUnroll
+// frame_sums: per block kFrameSumRows luma rows and half as many chroma rows of one job (luma outside the area the
+// window-stats kernel sums).  A lane takes kSumUnroll
 // columns 256 apart (clamped to the row, the surplus ones counted as zero) and walks down the block's rows two at a
 // time, so that a wave keeps many loads in flight.  Planes are read as buffers: a column's byte offset is fixed for
 // the walk and a row's offset is one scalar.  The sample sizes of the two pictures are template parameters, chosen
@@ -266,7 +267,9 @@ __device__ __forceinline__ void frame_sums_block(const FrameSumJob &J, int r0)
         const unsigned long long nr = (unsigned long long)(r1 - 1) * J.ry.pitch + w, nd = (unsigned long long)(r1 - 1) * J.dy.pitch + w;
         const SumPlane<BR> R(J.ry.ptr, J.ry.pitch, nr, J.ry.rshift, J.ry.shift);
         const SumPlane<BD> D(J.dy.ptr, J.dy.pitch, nd, J.dy.rshift, J.dy.shift);
-        for (int xb = threadIdx.x; xb < w; xb += 256 * U) {
+        // the block's rows lie all inside or all below the window-stats area, whose columns [0, lw) are summed there
+        const int lx0 = r0 < J.lh ? J.lw : 0;
+        for (int xb = lx0 + (int)threadIdx.x; xb < w; xb += 256 * U) {
             unsigned cr[U], cd[U];
             bool in[U];
 #pragma unroll

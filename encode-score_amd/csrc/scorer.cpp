@@ -810,9 +810,15 @@ private:
                 if (!on) continue;
                 w.r = src_plane(src[i], c);
                 w.d = enc_plane(encp[(size_t)i * E + e]);
-                if (chroma == 1)
-                    fj[(size_t)i * E + e] = frame_job(src[i], encp[(size_t)i * E + e], e, color_encode(c, f) == e,
-                                                      fsums.as<unsigned long long>() + ((size_t)i * E + e) * kFrameSums);
+                if (chroma == 1) {
+                    // the window-stats kernel adds the luma squared error of the area whole windows cover
+                    FrameSumJob &fo = fj[(size_t)i * E + e];
+                    fo = frame_job(src[i], encp[(size_t)i * E + e], e, color_encode(c, f) == e,
+                                   fsums.as<unsigned long long>() + ((size_t)i * E + e) * kFrameSums);
+                    fo.lw = cr.w / cfg.win * cfg.win;
+                    fo.lh = cr.h / cfg.win * cfg.win;
+                    w.sse = fo.out;
+                }
                 w.w = cr.w;
                 w.h = cr.h;
                 w.up = up;
