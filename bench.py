@@ -174,6 +174,9 @@ def main():
                     help="frames carry chroma planes: PSNR and mean colours in every step (1) or luma only (0)")
     ap.add_argument("--cache-run", type=int, default=1,
                     help="also time a second run whose source CAMBI comes from the source-CAMBI cache")
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="timed runs of the batched form, each a fresh scorer with the same warm-up and frame steps; "
+                         "value is the first, the line also gives all of them and their median")
     ap.add_argument("--ingest", type=int, default=50,
                     help="time this many launches of the frame-ingest kernel on a P010 frame of the source's size")
     args = ap.parse_args()
@@ -310,6 +313,14 @@ def main():
     stage = {k: stage[k] - warm_stage[k] for k in stage}
     rounds -= warm_rounds
     frame_steps_per_s = timed_frames / wall
+    # the repeats: the same settings, warm-up and timed frame steps, each on a scorer of its own
+    repeat_values = [round(frame_steps_per_s, 3)]
+    for _ in range(max(args.repeats, 1) - 1):
+        again = batched(profile=True, serial_stages=bool(args.serial))
+        submit(again, 0, warm)
+        again.sync()
+        repeat_values.append(round(timed_frames / timed(again, warm, timed_frames)[0], 3))
+        del again
     encode_frames = sum(min(length, total) - min(length, warm) for length in lengths)
     print(f"[bench] batched: {frame_steps_per_s:.1f} frame steps/s ({wall * 1e3 / args.steps:.3f} ms per step of "
           f"{batch})", file=sys.stderr, flush=True)
@@ -431,6 +442,8 @@ def main():
                    ("energies overlapped from the pyramid" if early else "energies overlapped from the c-values"),
                    "form": "batched, integer energies, graph replay"},
         "autotuner": tuned,
+        "repeats": {"frame_steps_per_s": repeat_values, "median": float(np.median(repeat_values)),
+                    "runs": "each a scorer of its own with the same warm-up and timed frame steps; value is the first"},
         "encode_frames_per_s": round(encode_frames / wall, 3),
         "device_ms_per_step": round(dev * 1e3 / args.steps, 4),
         "hbm_frac_b_in": round(step_gbs / HBM_PEAK_GBS, 6),
