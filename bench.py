@@ -123,7 +123,7 @@ def load_oracle():
     return mod
 
 
-TUNE_BATCH_MAX = 32               # the largest batch the autotuner tries
+TUNE_BATCH_MAX = 64               # the largest batch the autotuner tries
 TUNE_FRAMES = 3 * TUNE_BATCH_MAX  # it times three batches of it
 PLAIN_WARM = 16                   # untimed frame steps of the plain form (lasting texture starts at frame 12)
 

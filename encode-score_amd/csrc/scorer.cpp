@@ -1409,7 +1409,9 @@ int es_scorer_autotune(const es_scorer_config *cfg, int nframes, const void *con
         // the persistent waves' queues: one shared queue (0) or one per XCD (1)
         if (!best.cv_tile_blocks) try_set([](es_tune_result &t, int v) { t.cv_queues = v; }, {0, 1});
         if (!base.serial_stages) try_set([](es_tune_result &t, int v) { t.early_energies = v; }, {0, 1});
-        try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32});
+        // larger batches spread each round's launches and its c-value kernel's tail over more frames; a batch needs three
+        // batches of the caller's frames to be tried
+        try_set([](es_tune_result &t, int v) { t.batch = v; }, {8, 16, 32, 64});
         const int trials = out->trials;
         *out = best;
         out->trials = trials;

@@ -214,7 +214,7 @@ def test_tile_height_is_bounded(gpu):
 
 
 def test_autotune_picks_valid_settings_and_caches(gpu, tmp_path):
-    n = 96  # three batches of the largest batch tried
+    n = 96  # three batches of 32: batch 64 needs 192 frames and is not tried
     src, enc = frames_of(gpu, n, seed=12)
     ds = [gpu.DeviceArray.from_host(f) for f in src]
     de = [gpu.DeviceArray.from_host(f) for f in enc]
