@@ -129,10 +129,11 @@ PLAIN_WARM = 16                   # untimed frame steps of the plain form (lasti
 
 
 def frame_pool(steps, warmup, tune, batch, cap):
-    """Distinct frames generated per file: the autotuner's frames at least, and up to `cap` the run's frame steps
-    (the largest batch the run may take times its steps); later frame steps reuse the pool's frames in turn."""
+    """Distinct frames generated per file: up to `cap` the run's frame steps (the largest batch the run may take times
+    its steps), and with the autotuner its frames at least; later frame steps reuse the pool's frames in turn."""
     most_batch = TUNE_BATCH_MAX if tune else batch
-    return max(TUNE_FRAMES, min(cap, (warmup + steps) * most_batch))
+    run = min(cap, (warmup + steps) * most_batch)
+    return max(TUNE_FRAMES, run) if tune else run
 
 
 def main():

@@ -249,7 +249,7 @@ typedef struct {
     int win;          /* window of the energies: 16 (reference), 32 or 64 */
     int per_segment;  /* frames per segment (src/main.cpp:434) */
     int window_base;  /* CAMBI window base constant, 65 (src/cambi.cpp:39) */
-    int batch;        /* frames per batch in the batched form */
+    int batch;        /* frames per batch in the batched form (1 to 128; 0 = 16) */
     int tile_h;       /* c-value tile height, 0 = default */
     int use_graph;    /* batched form: capture the per-batch pipeline once and replay it */
     int n_encodes;

@@ -70,18 +70,20 @@ def test_scorer_matches_oracle(gpu, orc, mode):
     compare_segment(got, orc.accum_summary(run.merged()))
 
 
-def test_scorer_matches_oracle_at_the_largest_batch(gpu, orc):
-    """Rounds of 64 frames (the largest batch, the one the autotuner picks on every benchmark workload): two full
-    rounds and a partial one, submitted in pieces that straddle them, in graph replay with the matrix-core c-values."""
-    n, per_segment = 150, 40
-    print("case scorer batch 64", flush=True)
+@pytest.mark.parametrize("batch", [64, 128])
+def test_scorer_matches_oracle_at_the_largest_batch(gpu, orc, batch):
+    """Rounds of 64 and of 128 frames (the largest batch; the autotuner picks 64 to 128 on the benchmark workloads): two
+    full rounds and a partial one, submitted in pieces that straddle them, in graph replay with the matrix-core
+    c-values."""
+    n, per_segment = 2 * batch + 22, 40
+    print(f"case scorer batch {batch}", flush=True)
     src, enc = frames_of(gpu, n, seed=64)
     run, cam, cam_src = oracle_run(orc, src, enc, per_segment)
     ds = [gpu.DeviceArray.from_host(f) for f in src]
     de = [gpu.DeviceArray.from_host(f) for f in enc]
-    sc = gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], per_segment=per_segment, batch=64,
+    sc = gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], per_segment=per_segment, batch=batch,
                     use_graph=True, cv_mfma=True, tile_h=64)
-    for a0, a1 in ((0, 70), (70, 131), (131, n)):
+    for a0, a1 in ((0, batch + 6), (batch + 6, 2 * batch + 3), (2 * batch + 3, n)):
         sc.submit([d.ptr for d in ds[a0:a1]], [[d.ptr] for d in de[a0:a1]])
     sc.sync()
     a, b = sc.cambi(0, n)
@@ -93,8 +95,8 @@ def test_scorer_matches_oracle_at_the_largest_batch(gpu, orc):
         compare_segment(got, ref)
         assert np.array_equal(hist, ref[1])
     compare_segment(sc.segment(0, -1), orc.accum_summary(run.merged()))
-    with pytest.raises(gpu.EsError):  # the scorer takes at most 64 frames per batch
-        gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], batch=65)
+    with pytest.raises(gpu.EsError):  # the scorer takes at most 128 frames per batch
+        gpu.Scorer(gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(W, H, n, W, 2, 0)], batch=129)
 
 
 def test_batched_uneven_lengths_and_crops(gpu, orc):
