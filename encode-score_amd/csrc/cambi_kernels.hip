@@ -1164,10 +1164,12 @@ __device__ void cv_tile(Sh &sh, const CambiPic &pic, int s, int local)
     }
 }
 
-// Waves per SIMD each layout is built for: four for the matrix-core form over 32 numbers (10 KB of LDS: 16 waves per
-// CU), three otherwise.
+// Waves per SIMD each layout is built for, as its LDS allows: four for the matrix-core form over 32 numbers (10 KB:
+// 16 waves per CU), two for the 64-number layouts, three for the prefix-scan form over 32 numbers.
 template <typename Sh> constexpr int kCvWaves = 3;
 template <> constexpr int kCvWaves<CvSharedM<1>> = 4;
+template <> constexpr int kCvWaves<CvSharedM<2>> = 2;
+template <> constexpr int kCvWaves<CvShared<64>> = 2;
 
 // The tile of claim v on XCD queue k, or -1 when the queue is empty: queue k holds the k-th eighth of every scale's
 // tiles (tile_first lists the slots by scale, coarsest first, np pictures each), coarsest scale first.
