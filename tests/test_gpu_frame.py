@@ -170,9 +170,13 @@ def submit_formats(gpu, sc, src, enc, src_fmt, enc_fmt, junk=None, chunks=(9,), 
 
 
 @pytest.mark.parametrize("mode", ["plain", "batched", "graph"])
-def test_scorer_frames_with_chroma_match_oracle(gpu, orc, mode):
-    n, per_segment, w, h = 17, 7, 224, 128
-    print(f"case scorer frames {mode}", flush=True)
+@pytest.mark.parametrize("size", [(224, 128), (226, 134)])
+def test_scorer_frames_with_chroma_match_oracle(gpu, orc, mode, size):
+    """226 x 134: columns and rows outside whole 16 x 16 windows, whose luma squared error the batched form's frame sums
+    add beside the window-stats kernel's share."""
+    n, per_segment = 17, 7
+    w, h = size
+    print(f"case scorer frames {mode} {w}x{h}", flush=True)
     src, enc, crop = yuv_run(gpu, n, w, h, seed=41)
     run, cam, cam_src, ps = oracle_yuv(orc, src, enc, crop, w, h, per_segment)
     sc = gpu.Scorer(gpu.ES_PLAIN if mode == "plain" else gpu.ES_BATCHED, W, H, [gpu.EncodeDesc(w, h, n, w, 2, 0)],
